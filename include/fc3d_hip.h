@@ -1,0 +1,128 @@
+/* fc3d_hip.h — C-ABI of the MI355X (gfx950) path: plain pointers and sizes,
+ * no HIP or torch types in the signatures.
+ *
+ * Host arrays are in the caller's contact order ("original order").  A
+ * problem handle keeps W, q, mu, the reaction and the velocity resident in
+ * HBM in the colour-major slot order described in the library's design notes;
+ * set/get functions permute on the way in and out.
+ *
+ * Every function returns 0 on success or a negative NUMERICS_ERR_* code
+ * (NUMERICS_ERR_DEVICE when the HIP runtime or the device fails); the message
+ * is available from fc3d_hip_last_error().  Nothing here falls back to the CPU.
+ */
+#ifndef FC3D_HIP_H
+#define FC3D_HIP_H
+
+#include <stdint.h>
+
+#include "fc3d_numerics.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Parameters of one coloured NSGS solve.  The first block mirrors the
+ * iparam/dparam entries the reference fc3d_nsgs reads (fc3d_nsgs.c:560-950,
+ * onecontact...c:1017-1056, fc3d_projection.c:350-514); the second block holds
+ * the additions of this library. */
+typedef struct fc3d_hip_params {
+  int max_iter;              /* iparam[SICONOS_IPARAM_MAX_ITER]                   */
+  double tol;                /* dparam[SICONOS_DPARAM_TOL]                        */
+  int error_evaluation;      /* iparam[13]: FULL, LIGHT, LIGHT_WITH_FULL_FINAL, ADAPTIVE */
+  int error_frequency;       /* iparam[14]                                        */
+  int filter_local_solution; /* iparam[16]                                        */
+  int relaxation;            /* iparam[10]                                        */
+  double omega;              /* dparam[10]                                        */
+  int local_solver;          /* internal solver id: 565, 551, 550, 553, 552       */
+  int local_max_iter;        /* local iparam[0]                                   */
+  double local_tol;          /* local dparam[0]                                   */
+  int local_formulation;     /* local iparam[10]: AC STD or JeanMoreau STD        */
+  int rho_strategy;          /* local iparam[9]                                   */
+  double rho_constant;       /* local dparam[3]                                   */
+  int ls_max_iter;           /* local iparam[12]                                  */
+  int hybrid_strategy;       /* local iparam[14]                                  */
+  int hybrid_max_loop;       /* local iparam[15]                                  */
+  int hybrid_max_iter;       /* local iparam[16]                                  */
+  /* additions */
+  int precision;             /* 0: FP64 sweeps; 1: FP32 sweeps + FP64 refinement  */
+  int warm_start;            /* 0: resident reaction; 1: zero; 2: learned model   */
+  int schedule;              /* 0: autotuned; >0 explicit schedule id             */
+  int sweeps_per_launch;     /* sweeps captured per graph replay (>=1)            */
+  double fp32_tol;           /* FP32 phase hands over when its light error < this */
+  double fp32_local_tol;     /* local tolerance used by FP32 sweeps               */
+  int instrument;            /* 1: time each sweep-kernel launch with HIP events  */
+} fc3d_hip_params;
+
+typedef struct fc3d_hip_stats {
+  int info;              /* reference info: 0, 1, or NUMERICS_ERR_MAX_ITER      */
+  int iterations;        /* sweeps done (FP32 + FP64)                           */
+  int fp32_sweeps;       /* of which in FP32                                    */
+  double residual;       /* final fc3d_compute_error value (FP64)               */
+  double incr_error;     /* last light error                                    */
+  int num_colours;
+  int warm_start_used;   /* 0 resident, 1 zero, 2 learned                       */
+  double solve_ms;       /* device time from first sweep to final residual      */
+  double sweep_kernel_ms;     /* instrumented: summed sweep-kernel durations    */
+  int64_t sweep_kernel_launches;
+  double sweep_kernel_bytes;  /* algorithmic bytes moved by those launches      */
+} fc3d_hip_stats;
+
+FC3D_API int fc3d_hip_device_count(void);
+FC3D_API const char* fc3d_hip_last_error(void);
+/* Defaults equal to solver_options_create(SICONOS_FRICTION_3D_NSGS) with the
+ * hybrid local solver (fc3d_nsgs.c:919-950, onecontact...c:1037-1056). */
+FC3D_API void fc3d_hip_params_default(fc3d_hip_params* p);
+
+/* ---- resident problem ---------------------------------------------------- */
+typedef struct fc3d_hip_problem fc3d_hip_problem;
+
+/* Uploads W (CSR over 3x3 blocks, column-major values, diagonal block present
+ * in every row), q and mu; colours the contacts on the GPU and builds the
+ * colour-major sliced layout.  Replaces the per-call set-up of fc3d_nsgs
+ * (fc3d_nsgs.c:560-630) and of its local solver (onecontact...c:62-187). */
+FC3D_API int fc3d_hip_problem_create(int nc, const int64_t* row_ptr, const int64_t* col_idx,
+                                     const double* blocks, const double* q, const double* mu,
+                                     int device, fc3d_hip_problem** out);
+FC3D_API void fc3d_hip_problem_destroy(fc3d_hip_problem* h);
+FC3D_API int fc3d_hip_problem_num_colours(const fc3d_hip_problem* h);
+FC3D_API int64_t fc3d_hip_problem_nnzb(const fc3d_hip_problem* h);
+/* colour of each contact (original order) */
+FC3D_API int fc3d_hip_problem_colours(const fc3d_hip_problem* h, int* colour);
+/* sweep order: the original index of the contact at each sweep position */
+FC3D_API int fc3d_hip_problem_order(const fc3d_hip_problem* h, int* order);
+FC3D_API int fc3d_hip_set_reaction(fc3d_hip_problem* h, const double* reaction);
+FC3D_API int fc3d_hip_get_reaction(const fc3d_hip_problem* h, double* reaction);
+FC3D_API int fc3d_hip_get_velocity(const fc3d_hip_problem* h, double* velocity);
+
+/* fc3d_nsgs.c:560-917 in colour order on the resident reaction. */
+FC3D_API int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats* s);
+/* fc3d_compute_error.c:70-122 on the resident reaction; writes the resident
+ * velocity.  norm < 0 uses the stored |q|. */
+FC3D_API int fc3d_hip_compute_error(fc3d_hip_problem* h, double norm, double* error);
+/* SparseBlockMatrix.c:360-419: y += W x (host vectors, original order). */
+FC3D_API int fc3d_hip_gemv(fc3d_hip_problem* h, const double* x, double* y);
+
+/* ---- unit kernels over independent contacts ------------------------------ */
+/* fc3d_AlartCurnier_functions.c:506-753; formulation as local iparam[10]. */
+FC3D_API int fc3d_hip_alart_curnier(int n, const double* R, const double* U, const double* mu,
+                                    const double* rho, int formulation, double* F, double* A,
+                                    double* B);
+/* projectionOnCone.c:32-50, in place. */
+FC3D_API int fc3d_hip_projection_on_cone(int n, double* r, const double* mu);
+/* One local solve per contact (onecontact...c:221-290, 920-1015;
+ * fc3d_projection.c:350-603).  state: 4 doubles per contact [PLI rho, AC
+ * rho x3] in/out (initialised from W when init_state != 0). */
+FC3D_API int fc3d_hip_local_solve(int n, const double* W, const double* q, const double* mu,
+                                  double* r, const fc3d_hip_params* p, double* state,
+                                  int init_state, int* info);
+
+/* ---- colouring ----------------------------------------------------------- */
+/* Smallest colour not used by a lower-index neighbour, in index order;
+ * returns the number of colours. */
+FC3D_API int fc3d_hip_colour(int nc, const int64_t* row_ptr, const int64_t* col_idx,
+                             int* colour);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

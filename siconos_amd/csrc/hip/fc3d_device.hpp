@@ -1,0 +1,534 @@
+// fc3d_device.hpp — per-contact device math of the fc3d NSGS path (gfx950).
+//
+// Every function restates one reference function; the operation order is the
+// reference's, so that a build with -ffp-contract=off gives the same FP64
+// results as the reference C.  T is the sweep precision (double or float);
+// the 3x3 Newton factorisation always runs in double.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+namespace fc3d {
+
+template <typename T> struct num;
+template <> struct num<double> { static constexpr double eps = DBL_EPSILON; };
+template <> struct num<float> { static constexpr float eps = FLT_EPSILON; };
+
+// op3x3.h:342-363 — r += a v, column by column
+template <typename T>
+__device__ __forceinline__ void mvp3x3(const T* a, const T* v, T* r) {
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    r[0] += a[3 * j + 0] * v[j];
+    r[1] += a[3 * j + 1] * v[j];
+    r[2] += a[3 * j + 2] * v[j];
+  }
+}
+
+// op3x3.h:449-465 — c = a b
+template <typename T>
+__device__ __forceinline__ void mm3x3(const T* a, const T* b, T* c) {
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      c[i + 3 * j] = a[i + 0] * b[0 + 3 * j] + a[i + 3] * b[1 + 3 * j] + a[i + 6] * b[2 + 3 * j];
+}
+
+// op3x3.h:868-981 — partial-pivot 3x3 solve, 0 on success
+__device__ __forceinline__ int solve_3x3_gepp(const double* a, double* b) {
+  double lp0, lp1, lp2, lm1, lm2, ln1, ln2, bl, bm, bn, f1, f2, s0, s1, s2, alp0;
+  const double a0 = a[0], a1 = a[1], a2 = a[2];
+  const double aa0 = fabs(a0), aa1 = fabs(a1), aa2 = fabs(a2);
+  const int p1 = aa0 >= aa1 ? (aa0 >= aa2 ? 0 : 2) : (aa1 >= aa2 ? 1 : 2);
+  if (p1 == 0) {
+    f1 = a1 / a0; f2 = a2 / a0; lp0 = a0; alp0 = fabs(a0); lp1 = a[3]; lp2 = a[6];
+    lm1 = a[4] - f1 * lp1; lm2 = a[7] - f1 * lp2; ln1 = a[5] - f2 * lp1; ln2 = a[8] - f2 * lp2;
+    bl = b[0]; bm = b[1] - f1 * bl; bn = b[2] - f2 * bl;
+  } else if (p1 == 1) {
+    f1 = a0 / a1; f2 = a2 / a1; lp0 = a1; alp0 = fabs(a1); lp1 = a[4]; lp2 = a[7];
+    lm1 = a[3] - f1 * lp1; lm2 = a[6] - f1 * lp2; ln1 = a[5] - f2 * lp1; ln2 = a[8] - f2 * lp2;
+    bl = b[1]; bm = b[0] - f1 * bl; bn = b[2] - f2 * bl;
+  } else {
+    f1 = a0 / a2; f2 = a1 / a2; lp0 = a2; alp0 = fabs(a2); lp1 = a[5]; lp2 = a[8];
+    lm1 = a[3] - f1 * lp1; lm2 = a[6] - f1 * lp2; ln1 = a[4] - f2 * lp1; ln2 = a[7] - f2 * lp2;
+    bl = b[2]; bm = b[0] - f1 * bl; bn = b[1] - f2 * bl;
+  }
+  if (alp0 <= DBL_EPSILON) return 1;
+  const double alm1 = fabs(lm1), aln1 = fabs(ln1);
+  if (alm1 >= aln1) {
+    if (alm1 < DBL_EPSILON) return 1;
+    f1 = ln1 / lm1;
+    s2 = (bn - f1 * bm) / (ln2 - f1 * lm2);
+    s1 = (bm - lm2 * s2) / lm1;
+  } else {
+    if (aln1 < DBL_EPSILON) return 1;
+    f1 = lm1 / ln1;
+    s2 = (bm - f1 * bn) / (lm2 - f1 * ln2);
+    s1 = (bn - ln2 * s2) / ln1;
+  }
+  s0 = (bl - s1 * lp1 - s2 * lp2) / lp0;
+  b[0] = s0; b[1] = s1; b[2] = s2;
+  return 0;
+}
+
+// projectionOnCone.c:32-50
+template <typename T>
+__device__ __forceinline__ unsigned projection_on_cone(T* r, T mu) {
+  const T nt = sqrt(r[1] * r[1] + r[2] * r[2]);
+  if (mu * nt <= -r[0]) {
+    r[0] = T(0); r[1] = T(0); r[2] = T(0);
+    return 0u;
+  } else if (nt <= mu * r[0]) {
+    return 1u;
+  } else {
+    const T mu2 = mu * mu;
+    r[0] = (mu * nt + r[0]) / (mu2 + T(1));
+    r[1] = mu * r[0] * r[1] / nt;
+    r[2] = mu * r[0] * r[2] / nt;
+    return 2u;
+  }
+}
+
+// fc3d_AlartCurnier_functions.c:506-646 (Alart-Curnier, radius mu*max(0,RVN))
+template <typename T, bool AB>
+__device__ __forceinline__ void ac_std(const T* R, const T* u, T mu, const T* rho, T* F, T* A,
+                                       T* B) {
+  const T RhoN = rho[0], RhoT = rho[1];
+  const T RVN = R[0] - RhoN * u[0];
+  const T RVT = R[1] - RhoT * u[1];
+  const T RVS = R[2] - RhoT * u[2];
+  const T RV = sqrt(RVT * RVT + RVS * RVS);
+  T Radius;
+  if (AB) { A[3] = T(0); A[6] = T(0); A[1] = T(0); A[2] = T(0);
+            B[3] = T(0); B[6] = T(0); B[1] = T(0); B[2] = T(0); }
+  if (RVN > T(0)) {
+    Radius = mu * RVN;
+    F[0] = RhoN * u[0];
+    if (AB) { A[0] = RhoN; B[0] = T(0); }
+  } else {
+    Radius = T(0);
+    F[0] = R[0];
+    if (AB) { A[0] = T(0); B[0] = T(1); }
+  }
+  if (RV <= Radius) {
+    F[1] = RhoT * u[1];
+    F[2] = RhoT * u[2];
+    if (AB) {
+      A[4] = RhoT; A[7] = T(0); A[5] = T(0); A[8] = RhoT;
+      B[1] = T(0); B[4] = T(0); B[7] = T(0); B[2] = T(0); B[5] = T(0); B[8] = T(0);
+    }
+  } else if (RV > Radius) {
+    if (Radius > T(0)) {
+      const T RV1 = T(1) / RV;
+      F[1] = R[1] - Radius * RVT * RV1;
+      F[2] = R[2] - Radius * RVS * RV1;
+      if (AB) {
+        const T RV3 = RV1 * RV1 * RV1;
+        const T GTT = RV1 - RVT * RVT * RV3;
+        const T GTS = -RVT * RVS * RV3;
+        const T GST = GTS;
+        const T GSS = RV1 - RVS * RVS * RV3;
+        A[1] = mu * RhoN * RVT * RV1;
+        A[2] = mu * RhoN * RVS * RV1;
+        A[4] = GTT * RhoT * Radius;
+        A[7] = GTS * RhoT * Radius;
+        A[5] = GST * RhoT * Radius;
+        A[8] = GSS * RhoT * Radius;
+        B[1] = -mu * RVT * RV1;
+        B[4] = T(1) - GTT * Radius;
+        B[7] = -GTS * Radius;
+        B[2] = -mu * RVS * RV1;
+        B[5] = -GST * Radius;
+        B[8] = T(1) - GSS * Radius;
+      }
+    } else {
+      F[1] = R[1];
+      F[2] = R[2];
+      if (AB) {
+        A[4] = T(0); A[7] = T(0); A[5] = T(0); A[8] = T(0);
+        B[1] = T(0); B[4] = T(1); B[7] = T(0); B[2] = T(0); B[5] = T(0); B[8] = T(1);
+      }
+    }
+  }
+}
+
+// fc3d_AlartCurnier_functions.c:649-753 (radius mu*R[0])
+template <typename T, bool AB>
+__device__ __forceinline__ void ac_jeanmoreau(const T* R, const T* u, T mu, const T* rho, T* F,
+                                              T* A, T* B) {
+  const T RhoN = rho[0], RhoT = rho[1];
+  const T RVN = R[0] - RhoN * u[0];
+  const T RVT = R[1] - RhoT * u[1];
+  const T RVS = R[2] - RhoT * u[2];
+  const T RV = sqrt(RVT * RVT + RVS * RVS);
+  const T Radius = mu * R[0];
+  if (RVN >= T(0)) {
+    F[0] = RhoN * u[0];
+    if (AB) { A[0] = RhoN; B[0] = T(0); }
+  } else {
+    F[0] = R[0];
+    if (AB) { A[0] = T(0); B[0] = T(1); }
+  }
+  if (RV < Radius || RV < T(1e-20)) {
+    F[1] = RhoT * u[1];
+    F[2] = RhoT * u[2];
+    if (AB) {
+      A[4] = RhoT; A[7] = T(0); A[5] = T(0); A[8] = RhoT;
+      B[1] = T(0); B[4] = T(0); B[7] = T(0); B[2] = T(0); B[5] = T(0); B[8] = T(0);
+    }
+  } else {
+    const T RV1 = T(1) / RV;
+    F[1] = R[1] - Radius * RVT * RV1;
+    F[2] = R[2] - Radius * RVS * RV1;
+    if (AB) {
+      const T RV3 = RV1 * RV1 * RV1;
+      const T GTT = RV1 - RVT * RVT * RV3;
+      const T GTS = -RVT * RVS * RV3;
+      const T GST = GTS;
+      const T GSS = RV1 - RVS * RVS * RV3;
+      A[4] = GTT * RhoT * Radius;
+      A[7] = GTS * RhoT * Radius;
+      A[5] = GST * RhoT * Radius;
+      A[8] = GSS * RhoT * Radius;
+      B[1] = -mu * RVT * RV1;
+      B[4] = T(1) - GTT * Radius;
+      B[7] = -GTS * Radius;
+      B[2] = -mu * RVS * RV1;
+      B[5] = -GST * Radius;
+      B[8] = T(1) - GSS * Radius;
+    }
+  }
+}
+
+template <typename T, bool AB>
+__device__ __forceinline__ void ac_function(int formulation, const T* R, const T* u, T mu,
+                                            const T* rho, T* F, T* A, T* B) {
+  if (formulation == 1)
+    ac_jeanmoreau<T, AB>(R, u, mu, rho, F, A, B);
+  else
+    ac_std<T, AB>(R, u, mu, rho, F, A, B);
+}
+
+// fc3d_AlartCurnier_functions.c:784-812 (quirk sw - W21 + W12 kept) and
+// 755-782 (_cond variant); strategy ids of FrictionContact_options.h:329-340.
+template <typename T>
+__device__ __forceinline__ void compute_rho(int strategy, T rho_constant, const T* W, T* rho) {
+  if (strategy == 0) {
+    rho[0] = rho[1] = rho[2] = rho_constant;
+    return;
+  }
+  const T sw = W[1 + 1 * 3] + W[2 + 2 * 3];
+  T dw = sw * sw - T(4) * (sw - W[2 + 1 * 3] + W[1 + 2 * 3]);
+  dw = dw > T(0) ? sqrt(dw) : T(0);
+  rho[0] = T(1) / W[0];
+  if (strategy == 2)
+    rho[1] = T(2) * (sw - dw) / ((sw + dw) * (sw + dw));
+  else
+    rho[1] = T(2) / (sw + dw);
+  rho[2] = rho[1];
+}
+
+// fc3d_compute_error.c:42-56 — returns the squared local term
+template <typename T>
+__device__ __forceinline__ T unitary_error(const T* r, const T* u, T mu) {
+  T w[3];
+  w[0] = r[0] - u[0] - mu * sqrt(u[1] * u[1] + u[2] * u[2]);
+  w[1] = r[1] - u[1];
+  w[2] = r[2] - u[2];
+  projection_on_cone<T>(w, mu);
+  w[0] = r[0] - w[0];
+  w[1] = r[1] - w[1];
+  w[2] = r[2] - w[2];
+  return w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+}
+
+template <typename T>
+struct LocalProblem {
+  T W[9];
+  T q[3];
+  T mu;
+};
+
+template <typename T>
+__device__ __forceinline__ void local_velocity(const LocalProblem<T>& lp, const T* r, T* v) {
+  v[0] = lp.q[0]; v[1] = lp.q[1]; v[2] = lp.q[2];
+  mvp3x3<T>(lp.W, r, v);
+}
+
+// order of fc3d_projection.c:430-432 and onecontact...c:624-626
+template <typename T>
+__device__ __forceinline__ void local_velocity_inline(const LocalProblem<T>& lp, const T* r,
+                                                      T* v) {
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+    v[i] = lp.W[i + 0 * 3] * r[0] + lp.q[i] + lp.W[i + 1 * 3] * r[1] + lp.W[i + 2 * 3] * r[2];
+}
+
+// onecontact...c:856-878
+template <typename T>
+__device__ __forceinline__ T compute_local_error(const LocalProblem<T>& lp, const T* r) {
+  const T nq = sqrt(lp.q[0] * lp.q[0] + lp.q[1] * lp.q[1] + lp.q[2] * lp.q[2]);
+  T v[3];
+  local_velocity<T>(lp, r, v);
+  T e = sqrt(unitary_error<T>(r, v, lp.mu));
+  if (fabs(nq) > num<T>::eps) e /= nq;
+  return e;
+}
+
+struct LocalParams {
+  int solver;          // 565, 551, 550, 553, 552
+  int newton_max_iter; // local iparam[0]
+  int formulation;
+  int ls_max_iter;
+  int hybrid_strategy;
+  int hybrid_max_loop;
+  int hybrid_max_iter;
+};
+
+// onecontact...c:562-708
+template <typename T>
+__device__ void line_search_gp(const LocalProblem<T>& lp, int formulation, T* t_opt, const T* r,
+                               const T* dR, const T* rho, int ls_max, T* F, const T* A,
+                               const T* B, T* v) {
+  T alpha = *t_opt;
+  const T inf = T(1e20);
+  T amin = T(0), amax = inf;
+  const T m1 = T(0.1), m2 = T(0.9);
+  const T q0 = T(0.5) * (F[0] * F[0] + F[1] * F[1] + F[2] * F[2]);
+  T AWpB[9], tmp[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      T s = T(0);
+#pragma unroll
+      for (int k = 0; k < 3; k++) s += A[i + 3 * k] * lp.W[k + 3 * j];
+      AWpB[i + 3 * j] = s + B[i + 3 * j];
+    }
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    T s = T(0);
+#pragma unroll
+    for (int j = 0; j < 3; j++) s += AWpB[i + 3 * j] * dR[j];
+    tmp[i] = s;
+  }
+  T dqdt0 = T(0);
+#pragma unroll
+  for (int i = 0; i < 3; i++) dqdt0 += F[i] * tmp[i];
+  for (int it = 0; it < ls_max; it++) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) tmp[i] = r[i] + alpha * dR[i];
+    local_velocity_inline<T>(lp, tmp, v);
+    ac_function<T, false>(formulation, tmp, v, lp.mu, rho, F, nullptr, nullptr);
+    const T q = T(0.5) * (F[0] * F[0] + F[1] * F[1] + F[2] * F[2]);
+    const T slope = (q - q0) / alpha;
+    const bool C1 = slope >= m2 * dqdt0;
+    const bool C2 = slope <= m1 * dqdt0;
+    if (C1 && C2) {
+      *t_opt = alpha;
+      return;
+    } else if (!C1) {
+      amin = alpha;
+    } else {
+      amax = alpha;
+    }
+    if (alpha < inf)
+      alpha = T(0.5) * (amin + amax);
+    else
+      alpha = T(10) * alpha;
+  }
+  *t_opt = alpha;
+}
+
+// onecontact...c:429-559 (damped = false) and 710-854 (damped = true)
+template <typename T>
+__device__ int nsn_newton(const LocalProblem<T>& lp, T* r, const LocalParams& P, T tol,
+                          const T* rho, bool damped, T* residual) {
+  const T nq = sqrt(lp.q[0] * lp.q[0] + lp.q[1] * lp.q[1] + lp.q[2] * lp.q[2]);
+  T nrel = T(1);
+  if (nq > num<T>::eps) nrel /= nq;
+  // A and B start at zero: the JeanMoreau function leaves the coupling
+  // entries untouched (fc3d_AlartCurnier_functions.c:649-753)
+  T dR[3] = {T(0), T(0), T(0)}, F[3], v[3];
+  T A[9] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+  T B[9] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+  local_velocity<T>(lp, r, v);
+  T t = T(1);
+  ac_function<T, false>(P.formulation, r, v, lp.mu, rho, F, nullptr, nullptr);
+  *residual = sqrt(F[0] * F[0] + F[1] * F[1] + F[2] * F[2]) * nrel;
+  if (*residual < tol) return 0;
+  for (int inew = 1; inew < P.newton_max_iter; ++inew) {
+    ac_function<T, true>(P.formulation, r, v, lp.mu, rho, F, A, B);
+    T AW[9];
+    mm3x3<T>(A, lp.W, AW);
+    double M[9], d[3];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      T x = AW[k] + B[k];
+      x = x * T(-1);
+      M[k] = double(x);
+    }
+    d[0] = double(F[0]); d[1] = double(F[1]); d[2] = double(F[2]);
+    const int info3 = solve_3x3_gepp(M, d);
+    if (info3) {
+      dR[0] = T(0); dR[1] = T(0); dR[2] = T(0);
+    } else {
+      dR[0] = T(d[0]); dR[1] = T(d[1]); dR[2] = T(d[2]);
+    }
+    if (damped) {
+      if (!info3) {
+        T t_opt = T(1);
+        line_search_gp<T>(lp, P.formulation, &t_opt, r, dR, rho, P.ls_max_iter, F, A, B, v);
+        t = t_opt;
+      }
+      r[0] = r[0] + t * dR[0];
+      r[1] = r[1] + t * dR[1];
+      r[2] = r[2] + t * dR[2];
+    } else {
+      r[0] += dR[0]; r[1] += dR[1]; r[2] += dR[2];
+    }
+    local_velocity<T>(lp, r, v);
+    ac_function<T, false>(P.formulation, r, v, lp.mu, rho, F, nullptr, nullptr);
+    *residual = sqrt(F[0] * F[0] + F[1] * F[1] + F[2] * F[2]) * nrel;
+    const T ndR = sqrt(dR[0] * dR[0] + dR[1] * dR[1] + dR[2] * dR[2]);
+    if (damped) {
+      if (info3) break;
+      if (*residual < tol) return 0;
+      if (ndR < T(10) * num<T>::eps) return 1;
+    } else {
+      if (*residual < tol) return 0;
+      if (ndR < T(10) * num<T>::eps) return 1;
+      if (info3) break;
+    }
+  }
+  return 1;
+}
+
+// fc3d_projection.c:350-514; *rho_state is the persistent per-contact step
+template <typename T>
+__device__ int pli_solve(const LocalProblem<T>& lp, T* r, int max_iter, T tol, T* rho_state,
+                         T* residual) {
+  T rho = *rho_state, rho_k = rho;
+  T vel[3] = {T(0), T(0), T(0)}, vel_k[3], r_k[3];
+  T err = T(1), err_k, a1 = T(0), a2 = T(0);
+  int it = 0;
+  const T tau = T(2.0 / 3.0), tauinv = T(3.0 / 2.0), L = T(0.9), Lmin = T(0.3);
+  const T mu = lp.mu;
+  while ((err > tol) && (it < max_iter)) {
+    it++;
+    err_k = err;
+    r_k[0] = r[0]; r_k[1] = r[1]; r_k[2] = r[2];
+    local_velocity_inline<T>(lp, r, vel_k);
+    int ls_it = 0;
+    bool success = false;
+    rho_k = rho / tau;
+    const T nUT = sqrt(vel_k[1] * vel_k[1] + vel_k[2] * vel_k[2]);
+    while (!success && (ls_it < 10)) {
+      rho_k = rho_k * tau;
+      r[0] = r_k[0] - rho_k * (vel_k[0] + mu * nUT);
+      r[1] = r_k[1] - rho_k * vel_k[1];
+      r[2] = r_k[2] - rho_k * vel_k[2];
+      projection_on_cone<T>(r, mu);
+      local_velocity_inline<T>(lp, r, vel);
+      a1 = sqrt((vel_k[0] - vel[0]) * (vel_k[0] - vel[0]) +
+                (vel_k[1] - vel[1]) * (vel_k[1] - vel[1]) +
+                (vel_k[2] - vel[2]) * (vel_k[2] - vel[2]));
+      a2 = sqrt((r_k[0] - r[0]) * (r_k[0] - r[0]) + (r_k[1] - r[1]) * (r_k[1] - r[1]) +
+                (r_k[2] - r[2]) * (r_k[2] - r[2]));
+      success = (rho_k * a1 <= L * a2);
+      ls_it++;
+    }
+    err = T(0) + unitary_error<T>(r, vel, mu);
+    if ((rho_k * a1 < Lmin * a2) && (err < err_k))
+      rho = rho_k * tauinv;
+    else
+      rho = rho_k;
+  }
+  *rho_state = rho;
+  *residual = err;
+  return err > tol ? 1 : 0;
+}
+
+// fc3d_projection.c:564-603
+template <typename T>
+__device__ __forceinline__ int proj_solve(const LocalProblem<T>& lp, T* r) {
+  const T an = T(1) / lp.W[0];
+  T w[3];
+  local_velocity_inline<T>(lp, r, w);
+  const T nUT = sqrt(w[1] * w[1] + w[2] * w[2]);
+  r[0] -= an * (w[0] + lp.mu * nUT);
+  r[1] -= an * w[1];
+  r[2] -= an * w[2];
+  projection_on_cone<T>(r, lp.mu);
+  return 0;
+}
+
+// onecontact...c:880-918
+template <typename T>
+__device__ __forceinline__ void keep_or_discard(const LocalProblem<T>& lp, T* r, T* bak, T tol,
+                                                T residual, T* cur) {
+  const T e = compute_local_error<T>(lp, r);
+  if (isnan(residual) || isinf(residual)) {
+    r[0] = bak[0]; r[1] = bak[1]; r[2] = bak[2];
+  } else if (e <= tol || e <= *cur) {
+    *cur = e;
+    bak[0] = r[0]; bak[1] = r[1]; bak[2] = r[2];
+  } else {
+    r[0] = bak[0]; r[1] = bak[1]; r[2] = bak[2];
+  }
+}
+
+// onecontact...c:920-1015
+template <typename T>
+__device__ int hybrid_solve(const LocalProblem<T>& lp, T* r, const LocalParams& P, T tol,
+                            T* rho_pli, const T* rho, T* residual) {
+  int info = -1;
+  T bak[3] = {r[0], r[1], r[2]};
+  T cur = compute_local_error<T>(lp, r);
+  T res = T(0);
+  if (P.hybrid_strategy == 2) {
+    info = nsn_newton<T>(lp, r, P, tol, rho, true, &res);
+    keep_or_discard<T>(lp, r, bak, tol, res, &cur);
+    if (cur <= tol) {
+      *residual = cur;
+      return info;
+    }
+  }
+  int loop = 0;
+  while (loop < P.hybrid_max_loop && cur >= tol) {
+    loop++;
+    info = pli_solve<T>(lp, r, P.hybrid_max_iter, tol, rho_pli, &res);
+    keep_or_discard<T>(lp, r, bak, tol, res, &cur);
+    if (cur < tol) break;
+    info = nsn_newton<T>(lp, r, P, tol, rho, true, &res);
+    keep_or_discard<T>(lp, r, bak, tol, res, &cur);
+  }
+  *residual = cur;
+  return info;
+}
+
+// onecontact...c:221-290 (take-off, then dispatch) and the projection solvers.
+// state_pli: PLI step; rho: AC parameters; residual: SOLVER_RESIDUAL afterwards.
+template <typename T>
+__device__ int local_solve(const LocalProblem<T>& lp, T* r, const LocalParams& P, T tol,
+                           T* state_pli, const T* rho, T* residual) {
+  if (P.solver == 552) {
+    // the projection solver never sets SOLVER_RESIDUAL, which keeps the
+    // INFINITY given by solver_options_initialize (SolverOptions.c:74)
+    *residual = T(INFINITY);
+    return proj_solve<T>(lp, r);
+  }
+  if (P.solver == 553) return pli_solve<T>(lp, r, P.newton_max_iter, tol, state_pli, residual);
+  if (lp.q[0] > -num<T>::eps) {
+    r[0] = T(0); r[1] = T(0); r[2] = T(0);
+    *residual = T(0);
+    return 0;
+  }
+  if (P.solver == 550) return nsn_newton<T>(lp, r, P, tol, rho, false, residual);
+  if (P.solver == 551) return nsn_newton<T>(lp, r, P, tol, rho, true, residual);
+  return hybrid_solve<T>(lp, r, P, tol, state_pli, rho, residual);
+}
+
+}  // namespace fc3d

@@ -1,0 +1,194 @@
+"""The coloured NSGS on the GPU against the oracle run in the same sweep
+order (fc3d_nsgs.c:560-917).  FP64 results are compared bit for bit; the
+natural-order reference run is compared on the residual (W is rank
+deficient, so reactions are not unique)."""
+import numpy as np
+import pytest
+
+from conftest import load_fixture, natural_residual
+from siconos_amd import capi
+from siconos_amd import numerics as N
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_solve(W, q, mu, r0=None, **kw):
+    H = N.HipProblem(W, q, mu)
+    H.set_reaction(np.zeros(3 * W.nc) if r0 is None else r0)
+    s = H.solve(N.hip_params(**kw))
+    return H, s
+
+
+def test_colouring_matches_the_sequential_definition(oracle):
+    for g in (N.generate_random_pairs(300, 1000, 0.2, 0.5, 1),
+              N.generate_random_pairs(20000, 50000, 0.2, 0.5, 2),
+              N.generate_granular_pile(20000, 50000, 0.1, 0.9, 5)):
+        cg, ng = N.colour_contacts(g.W)
+        co, no = oracle.greedy_colouring(g.W)
+        np.testing.assert_array_equal(cg, co)
+        assert ng == no
+        assert N.colouring_is_valid(g.W, cg)
+
+
+def test_residual_and_gemv(oracle):
+    g = N.generate_random_pairs(300, 1000, 0.2, 0.5, 1)
+    H = N.HipProblem(g.W, g.q, g.mu)
+    rng = np.random.default_rng(0)
+    for t in range(3):
+        r = rng.normal(size=3000) * (t + 1)
+        H.set_reaction(r)
+        e = H.compute_error()
+        eo, uo = oracle.compute_error(g.W, g.q, g.mu, r)
+        np.testing.assert_array_equal(H.velocity(), uo)
+        np.testing.assert_allclose(e, eo, rtol=1e-14)
+        y0 = rng.normal(size=3000)
+        np.testing.assert_allclose(H.gemv(r, y0), y0 + g.W.to_dense() @ r, rtol=1e-13,
+                                   atol=1e-13)
+    # reaction round trip through the slot order
+    np.testing.assert_array_equal(H.reaction(), r)
+
+
+@pytest.mark.parametrize("cfg", ["cfg1", "cfg2", "pile50k"])
+def test_plain_fp64_bit_exact_with_oracle_in_colour_order(oracle, cfg):
+    if cfg == "cfg1":
+        g, tol = N.generate_random_pairs(300, 1000, 0.2, 0.5, 1), 1e-8
+    elif cfg == "cfg2":
+        g, tol = N.generate_random_pairs(20000, 50000, 0.2, 0.5, 2), 1e-6
+    else:
+        g, tol = N.generate_granular_pile(20000, 50000, 0.1, 0.9, 5), 1e-6
+    H, s = gpu_solve(g.W, g.q, g.mu, tol=tol, max_iter=1000)
+    assert s.info == 0 and s.residual < tol
+    info, ro, uo, it, res = oracle.nsgs(g.W, g.q, g.mu, tol=tol, max_iter=1000, order=H.order())
+    assert info == 0 and it == s.iterations
+    np.testing.assert_array_equal(H.reaction(), ro)
+    np.testing.assert_array_equal(H.velocity(), uo)
+    np.testing.assert_allclose(s.residual, res, rtol=1e-12)
+    # the natural-order reference reaches the same tolerance
+    info_n, rn, un, itn, resn = oracle.nsgs(g.W, g.q, g.mu, tol=tol, max_iter=1000)
+    assert info_n == 0 and resn < tol
+
+
+@pytest.mark.parametrize("variant", range(5))
+@pytest.mark.parametrize("name", ["Capsules-i122-1617.dat", "Capsules-i101-404.dat",
+                                  "NESpheres_10_1.dat", "Rover1039.dat", "Rover9770.dat",
+                                  "FrictionContact3D_1c.dat", "FC3D_Example1_SBM.dat",
+                                  "BoxesStack1-i100000-32.hdf5.dat", "Confeti-ex03-Fc3D-SBM.dat",
+                                  "OneObject-i1028-138.hdf5.dat", "Rover4396.dat"])
+def test_fixtures_through_fc3d_driver(oracle, name, variant):
+    """fc3d_driver with SICONOS_FRICTION_3D_NSGS_HIP and the five option sets
+    of the reference test/test_nsgs_1.c, against the oracle in colour order."""
+    from test_oracle_nsgs import _options
+    W, q, mu = load_fixture(name)
+    ref = _options(oracle, variant)
+    o = N.SolverOptions(540)
+    o.iparam[:20] = ref.iparam[:20]
+    o.dparam[:20] = ref.dparam[:20]
+    o.update_internal(0, ref.internal()._s.solverId)
+    o.internal().iparam[:] = ref.internal().iparam[:]
+    o.internal().dparam[:] = ref.internal().dparam[:]
+    p = N.FrictionContactProblem(3, W, q, mu)
+    r, u = np.zeros(3 * W.nc), np.zeros(3 * W.nc)
+    info = N.fc3d_driver(p, r, u, o)
+    H = N.HipProblem(W, q, mu)
+    ref2 = _options(oracle, variant)
+    info_o, ro, uo, it, res = oracle.nsgs(W, q, mu, options=ref2, order=H.order())
+    assert info == info_o
+    assert o.iparam[1] == it
+    np.testing.assert_array_equal(r, ro)
+    np.testing.assert_array_equal(u, uo)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_error_evaluation_modes(oracle, mode):
+    g = N.generate_random_pairs(300, 1000, 0.2, 0.5, 1)
+    H, s = gpu_solve(g.W, g.q, g.mu, tol=1e-7, max_iter=1000, error_evaluation=mode)
+    o = oracle.Options()
+    o.iparam[13] = mode
+    info, ro, uo, it, res = oracle.nsgs(g.W, g.q, g.mu, options=o, tol=1e-7, max_iter=1000,
+                                        order=H.order())
+    assert s.info == info == 0 and s.iterations == it
+    np.testing.assert_array_equal(H.reaction(), ro)
+    if mode == 1:
+        assert s.residual == np.inf and res == np.inf
+    else:
+        np.testing.assert_allclose(s.residual, res, rtol=1e-12)
+
+
+def test_max_iter_and_warm_start(oracle):
+    g = N.generate_random_pairs(300, 1000, 0.2, 0.5, 1)
+    H, s = gpu_solve(g.W, g.q, g.mu, tol=1e-8, max_iter=3)
+    assert s.info == -20 and s.iterations == 3
+    r3 = H.reaction()
+    # warm start from the 3-sweep iterate == 6 sweeps from zero, in the same order
+    s2 = H.solve(N.hip_params(tol=1e-8, max_iter=3))
+    info, ro, uo, it, res = oracle.nsgs(g.W, g.q, g.mu, tol=1e-8, max_iter=3, reaction=r3,
+                                        order=H.order())
+    np.testing.assert_array_equal(H.reaction(), ro)
+    # warm_start = 1 discards the resident reaction
+    s3 = H.solve(N.hip_params(tol=1e-8, max_iter=1000, warm_start=1))
+    info, ro, uo, it, res = oracle.nsgs(g.W, g.q, g.mu, tol=1e-8, max_iter=1000, order=H.order())
+    assert s3.iterations == it
+    np.testing.assert_array_equal(H.reaction(), ro)
+
+
+def test_graph_chunking_does_not_change_results():
+    g = N.generate_random_pairs(20000, 50000, 0.2, 0.5, 2)
+    H = N.HipProblem(g.W, g.q, g.mu)
+    out = []
+    for spl, instr in [(1, 0), (7, 0), (32, 0), (1, 1)]:
+        H.set_reaction(np.zeros(3 * g.nc))
+        s = H.solve(N.hip_params(tol=1e-6, max_iter=1000, sweeps_per_launch=spl,
+                                 instrument=instr))
+        out.append((s.iterations, H.reaction()))
+    for it, r in out[1:]:
+        assert it == out[0][0]
+        np.testing.assert_array_equal(r, out[0][1])
+
+
+def test_mixed_precision_reaches_the_fp64_tolerance():
+    """FP32 sweeps + FP64 refinement: the FP64 residual of the result is below
+    the tolerance; the reaction differs from the FP64 run by the stated
+    amount (W is rank deficient: reactions agree only up to the solution
+    set, velocities closer)."""
+    g = N.generate_random_pairs(20000, 50000, 0.2, 0.5, 2)
+    H = N.HipProblem(g.W, g.q, g.mu)
+    s64 = H.solve(N.hip_params(tol=1e-6, max_iter=1000, warm_start=1))
+    u64 = H.velocity()
+    s = H.solve(N.hip_params(tol=1e-6, max_iter=1000, warm_start=1, precision=1))
+    assert s.info == 0 and s.residual < 1e-6 and s.fp32_sweeps > 0
+    e, _ = natural_residual(g.W, g.q, g.mu, H.reaction())
+    assert e < 1e-6
+    du = np.linalg.norm(H.velocity() - u64) / np.linalg.norm(u64)
+    assert du < 1e-3
+
+
+def test_invalid_options_are_refused(lib):
+    g = N.generate_random_pairs(300, 1000, 0.2, 0.5, 1)
+    H = N.HipProblem(g.W, g.q, g.mu)
+    with pytest.raises(RuntimeError):
+        H.solve(N.hip_params(local_solver=561))
+    with pytest.raises(RuntimeError):
+        H.solve(N.hip_params(error_evaluation=7))
+    o = N.SolverOptions(540)
+    o.iparam[11] = 1  # shuffle: not supported by the coloured sweep
+    p = N.FrictionContactProblem(3, g.W, g.q, g.mu)
+    r, u = np.zeros(3000), np.zeros(3000)
+    assert N.fc3d_driver(p, r, u, o) == -5
+
+
+def test_problem_create_rejects_bad_structure():
+    g = N.generate_random_pairs(300, 1000, 0.2, 0.5, 1)
+    W = g.W
+    # drop the diagonal block of row 0
+    keep = np.ones(W.nnzb, dtype=bool)
+    k0 = [k for k in range(W.row_ptr[0], W.row_ptr[1]) if W.col_idx[k] == 0][0]
+    keep[k0] = False
+    rp = W.row_ptr.copy()
+    rp[1:] -= 1
+    bad = N.SparseBlock3(rp, W.col_idx[keep], W.values[keep])
+    with pytest.raises(RuntimeError, match="diagonal"):
+        N.HipProblem(bad, g.q, g.mu)
+    ci = W.col_idx.copy()
+    ci[5] = 10 ** 6
+    with pytest.raises(RuntimeError, match="out of range"):
+        N.HipProblem(N.SparseBlock3(W.row_ptr, ci, W.values), g.q, g.mu)

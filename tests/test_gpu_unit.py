@@ -1,0 +1,91 @@
+"""Device functions through the C-ABI, against the oracle (bit-exact in FP64:
+both sides round every product before the sum it feeds)."""
+import numpy as np
+import pytest
+
+from conftest import read_ac_inputs
+from siconos_amd import capi
+from siconos_amd import numerics as N
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("formulation,fn", [(0, "oracle_computeAlartCurnierSTD"),
+                                            (1, "oracle_computeAlartCurnierJeanMoreau")])
+def test_alart_curnier_kernel_on_acinputs(oracle, lib, formulation, fn):
+    R, U, MU, RHO = read_ac_inputs()
+    n = len(MU)
+    F, A, B = np.zeros(3 * n), np.zeros(9 * n), np.zeros(9 * n)
+    Rf, Uf, RHOf = (np.ascontiguousarray(x.reshape(-1)) for x in (R, U, RHO))
+    rc = lib.fc3d_hip_alart_curnier(n, N.dptr(Rf), N.dptr(Uf), N.dptr(MU), N.dptr(RHOf),
+                                    formulation, N.dptr(F), N.dptr(A), N.dptr(B))
+    assert rc == 0, capi.last_error()
+    L = oracle.load()
+    for k in range(n):
+        Fo, Ao, Bo = np.zeros(3), np.zeros(9), np.zeros(9)
+        getattr(L, fn)(oracle.d(Rf[3 * k:3 * k + 3].copy()), oracle.d(Uf[3 * k:3 * k + 3].copy()),
+                       MU[k], oracle.d(RHOf[3 * k:3 * k + 3].copy()), oracle.d(Fo), oracle.d(Ao),
+                       oracle.d(Bo))
+        np.testing.assert_array_equal(F[3 * k:3 * k + 3], Fo)
+        np.testing.assert_array_equal(A[9 * k:9 * k + 9], Ao)
+        np.testing.assert_array_equal(B[9 * k:9 * k + 9], Bo)
+
+
+def test_projection_kernel(oracle, lib):
+    rng = np.random.default_rng(21)
+    n = 5000
+    r = rng.normal(size=3 * n)
+    mu = rng.uniform(0.05, 1.2, size=n)
+    g = r.copy()
+    assert lib.fc3d_hip_projection_on_cone(n, N.dptr(g), N.dptr(mu)) == 0
+    L = oracle.load()
+    for k in range(n):
+        x = r[3 * k:3 * k + 3].copy()
+        L.oracle_projectionOnCone(oracle.d(x), mu[k])
+        np.testing.assert_array_equal(g[3 * k:3 * k + 3], x)
+
+
+def local_params_from(o, sid):
+    """C-ABI parameters equal to the oracle's local options (reference defaults)."""
+    return N.hip_params(local_solver=sid, local_max_iter=int(o.iparam[0]),
+                        local_tol=float(o.dparam[0]), local_formulation=int(o.iparam[10]),
+                        rho_strategy=int(o.iparam[9]), rho_constant=float(o.dparam[3]),
+                        ls_max_iter=int(o.iparam[12]), hybrid_strategy=int(o.iparam[14]),
+                        hybrid_max_loop=int(o.iparam[15]), hybrid_max_iter=int(o.iparam[16]))
+
+
+@pytest.mark.parametrize("sid", [565, 551, 550, 553, 552])
+def test_local_solve_kernel(oracle, lib, sid):
+    rng = np.random.default_rng(22 + sid)
+    n = 2000
+    Ws, qs, mus, r0 = [], [], [], []
+    for k in range(n):
+        M = rng.normal(size=(3, 3))
+        W = M @ M.T + rng.uniform(0.1, 3) * np.eye(3)
+        if k % 3 == 0:
+            W = np.diag([2.0, 7.0, 7.0])
+        Ws.append(W.reshape(-1, order="F"))
+        qs.append(rng.normal(size=3))
+        mus.append(rng.uniform(0.1, 0.9))
+        r0.append(rng.normal(size=3) * (k % 2))
+    W = np.ascontiguousarray(np.concatenate(Ws))
+    q = np.ascontiguousarray(np.concatenate(qs))
+    mu = np.array(mus)
+    r = np.ascontiguousarray(np.concatenate(r0))
+    o = oracle.Options.local(sid)
+    p = local_params_from(o, sid)
+    state = np.zeros(4 * n)
+    info = np.zeros(n, dtype=np.int32)
+    rg = r.copy()
+    assert lib.fc3d_hip_local_solve(n, N.dptr(W), N.dptr(q), N.dptr(mu), N.dptr(rg), p,
+                                    N.dptr(state), 1, N.iptr(info)) == 0, capi.last_error()
+    L = oracle.load()
+    for k in range(n):
+        st = np.zeros(4)
+        w = W[9 * k:9 * k + 9].copy()
+        L.oracle_local_state_init(oracle.d(w), o.p, oracle.d(st))
+        x = r[3 * k:3 * k + 3].copy()
+        L.oracle_local_solve(oracle.d(w), oracle.d(q[3 * k:3 * k + 3].copy()), mu[k],
+                             oracle.d(x), o.p, oracle.d(st))
+        np.testing.assert_array_equal(rg[3 * k:3 * k + 3], x, err_msg=f"contact {k}")
+        np.testing.assert_array_equal(state[4 * k:4 * k + 4], st)

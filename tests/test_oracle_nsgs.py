@@ -1,0 +1,138 @@
+"""The oracle's fc3d_compute_error and fc3d_nsgs (fc3d_compute_error.c:70-122,
+fc3d_nsgs.c:560-917) against closed forms, an independent residual, the
+reference fixtures under the option sets of test/test_nsgs_1.c, and the
+colour-order invariance that the GPU solver relies on."""
+import numpy as np
+import pytest
+
+from conftest import load_fixture, natural_residual
+from siconos_amd import numerics as N
+
+# files of the reference's data_collection_1.c that are committed here; the
+# five option sets of test_nsgs_1.c are expected to converge on all of them
+COLLECTION1 = ["FC3D_Example1_SBM.dat", "FC3D_Example1.dat", "FrictionContact3D_1c.dat",
+               "FrictionContact3D_RR_1c.dat", "Capsules-i122-1617.dat", "Capsules-i101-404.dat",
+               "NESpheres_10_1.dat", "Rover1039.dat", "Rover11035.dat", "Rover9770.dat"]
+
+
+def example1_solution():
+    s = np.sqrt(10.0)
+    r = np.tile([1.0, -0.1 / s, -0.3 / s], 3)
+    u = np.tile([0.0, 1 - 0.1 / s, 3 - 0.3 / s], 3)
+    return r, u
+
+
+@pytest.mark.parametrize("name", ["FC3D_Example1_SBM.dat", "FC3D_Example1.dat"])
+def test_example1_closed_form(oracle, name):
+    W, q, mu = load_fixture(name)
+    np.testing.assert_array_equal(W.to_dense(), np.eye(9))
+    r, u = example1_solution()
+    e, uo = oracle.compute_error(W, q, mu, r)
+    assert e < 1e-15
+    np.testing.assert_allclose(uo, u, rtol=1e-15)
+    info, rr, uu, it, res = oracle.nsgs(W, q, mu, tol=1e-12)
+    assert info == 0 and res < 1e-12
+    np.testing.assert_allclose(rr, r, rtol=1e-14)
+    np.testing.assert_allclose(uu, u, rtol=1e-14)
+
+
+def test_compute_error_matches_independent_residual(oracle):
+    g = N.generate_random_pairs(300, 1000, 0.2, 0.5, 1)
+    rng = np.random.default_rng(0)
+    for _ in range(3):
+        r = rng.normal(size=3000)
+        e, u = oracle.compute_error(g.W, g.q, g.mu, r)
+        e2, u2 = natural_residual(g.W, g.q, g.mu, r)
+        np.testing.assert_allclose(u, u2, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(e, e2, rtol=1e-10)
+
+
+@pytest.mark.parametrize("mut", [2, 3])
+def test_residual_mutations_are_caught(oracle, mut):
+    g = N.generate_random_pairs(300, 1000, 0.2, 0.5, 1)
+    r = np.random.default_rng(0).normal(size=3000)
+    e2, _ = natural_residual(g.W, g.q, g.mu, r)
+    try:
+        oracle.mutation(mut)
+        e, _ = oracle.compute_error(g.W, g.q, g.mu, r)
+    finally:
+        oracle.mutation(0)
+    assert abs(e - e2) > 1e-6 * e2
+
+
+def _options(oracle, variant):
+    """The five option sets of the reference test/test_nsgs_1.c:26-114."""
+    o = oracle.Options()
+    o.dparam[0] = 1e-5
+    if variant == 1:
+        o.set_local_solver(551)
+        o.internal().iparam[10] = 1  # JeanMoreau formulation
+    elif variant == 2:
+        o.iparam[0] = 10000
+        o.set_local_solver(553)
+        o.internal().dparam[0] = 1e-12
+        o.internal().iparam[0] = 10
+    elif variant == 3:
+        o.iparam[0] = 10000
+        o.set_local_solver(551)
+        o.internal().iparam[14] = 2
+    elif variant == 4:
+        o.iparam[0] = 10000
+        o.set_local_solver(551)
+        o.internal().iparam[14] = 1
+    return o
+
+
+@pytest.mark.parametrize("variant", range(5))
+@pytest.mark.parametrize("name", COLLECTION1)
+def test_fixture_collections_converge(oracle, name, variant):
+    W, q, mu = load_fixture(name)
+    o = _options(oracle, variant)
+    info, r, u, it, res = oracle.nsgs(W, q, mu, options=o)
+    assert info == 0, (name, variant, it, res)
+    assert np.all(np.isfinite(r)) and np.all(np.isfinite(u))
+    e, _ = natural_residual(W, q, mu, r)
+    assert e <= 1e-5 * (1 + 1e-6)
+
+
+def test_config1_converges_to_1e8(oracle):
+    """300 spheres, 1000 contacts, tol 1e-8, 1000 sweeps (the CPU case)."""
+    g = N.generate_random_pairs(300, 1000, 0.2, 0.5, 1)
+    info, r, u, it, res = oracle.nsgs(g.W, g.q, g.mu, tol=1e-8, max_iter=1000)
+    assert info == 0 and res < 1e-8
+    e, _ = natural_residual(g.W, g.q, g.mu, r)
+    assert e < 1e-8
+
+
+def test_order_inside_a_colour_does_not_matter(oracle):
+    """Contacts of one colour share no body (W_ij = 0), so any order inside a
+    colour gives the same reactions bit for bit."""
+    g = N.generate_random_pairs(300, 1000, 0.2, 0.5, 3)
+    colour, nco = oracle.greedy_colouring(g.W)
+    rng = np.random.default_rng(1)
+    orders = []
+    for t in range(2):
+        order = []
+        for c in range(nco):
+            members = np.flatnonzero(colour == c)
+            order.extend(rng.permutation(members) if t else members)
+        orders.append(np.array(order, dtype=np.int32))
+    out = [oracle.nsgs(g.W, g.q, g.mu, tol=1e-30, max_iter=40, order=o)[1] for o in orders]
+    np.testing.assert_array_equal(out[0], out[1])
+    natural = oracle.nsgs(g.W, g.q, g.mu, tol=1e-30, max_iter=40)[1]
+    assert not np.array_equal(natural, out[0])
+
+
+def test_max_iter_returns_minus_20(oracle):
+    g = N.generate_random_pairs(300, 1000, 0.2, 0.5, 1)
+    info, r, u, it, res = oracle.nsgs(g.W, g.q, g.mu, tol=1e-8, max_iter=1)
+    assert info == -20 and it == 1
+
+
+def test_trivial_case(oracle):
+    g = N.generate_random_pairs(300, 1000, 0.2, 0.5, 1)
+    q = np.abs(g.q)
+    info, r, u, it, res = oracle.nsgs(g.W, q, g.mu, tol=1e-8)
+    assert info == 0 and it == 0
+    np.testing.assert_array_equal(r, 0.0)
+    np.testing.assert_array_equal(u, q)
