@@ -46,7 +46,7 @@ typedef struct fc3d_hip_params {
   /* additions */
   int precision;             /* 0: FP64 sweeps; 1: FP32 sweeps + FP64 refinement  */
   int warm_start;            /* 0: resident reaction; 1: zero; 2: learned model   */
-  int schedule;              /* 0: autotuned; >0 explicit schedule id             */
+  int schedule;              /* 0: autotuned; 1..6 explicit sweep-kernel schedule */
   int sweeps_per_launch;     /* sweeps captured per graph replay (>=1)            */
   double fp32_tol;           /* FP32 phase hands over when its light error < this */
   double fp32_local_tol;     /* local tolerance used by FP32 sweeps               */
@@ -65,6 +65,7 @@ typedef struct fc3d_hip_stats {
   double sweep_kernel_ms;     /* instrumented: summed sweep-kernel durations    */
   int64_t sweep_kernel_launches;
   double sweep_kernel_bytes;  /* algorithmic bytes moved by those launches      */
+  int schedule;               /* sweep-kernel schedule used (autotuned when 0 given) */
 } fc3d_hip_stats;
 
 FC3D_API int fc3d_hip_device_count(void);
@@ -115,6 +116,38 @@ FC3D_API int fc3d_hip_projection_on_cone(int n, double* r, const double* mu);
 FC3D_API int fc3d_hip_local_solve(int n, const double* W, const double* q, const double* mu,
                                   double* r, const fc3d_hip_params* p, double* state,
                                   int init_state, int* info);
+
+/* ---- many independent problems in one persistent launch -------------------- */
+/* No reference counterpart (the reference solves one problem per fc3d_driver
+ * call); each problem follows fc3d_nsgs.c:560-917 in its own colour order. */
+typedef struct fc3d_hip_batch fc3d_hip_batch;
+
+typedef struct fc3d_hip_batch_stats {
+  double solve_ms;            /* device time of the persistent launch            */
+  int converged;              /* problems with final residual <= tol             */
+  int64_t converged_contacts; /* contacts of those problems                      */
+  int64_t total_sweeps;
+  int64_t fp32_sweeps;
+  double max_residual;
+  int schedule;               /* 1: one wave per problem, 2: four waves per problem */
+} fc3d_hip_batch_stats;
+
+/* Problem p owns contacts [sum(nc[0..p)), sum(nc[0..p])) of the concatenated
+ * CSR (row_ptr has sum(nc) + 1 entries; column indices are global and must
+ * stay inside their problem). */
+FC3D_API int fc3d_hip_batch_create(int nprob, const int* nc, const int64_t* row_ptr,
+                                   const int64_t* col_idx, const double* blocks, const double* q,
+                                   const double* mu, int device, fc3d_hip_batch** out);
+FC3D_API void fc3d_hip_batch_destroy(fc3d_hip_batch* b);
+/* sweep order of every problem, problems one after the other (global indices) */
+FC3D_API int fc3d_hip_batch_order(const fc3d_hip_batch* b, int* order);
+FC3D_API int fc3d_hip_batch_set_reaction(fc3d_hip_batch* b, const double* reaction);
+FC3D_API int fc3d_hip_batch_get_reaction(const fc3d_hip_batch* b, double* reaction);
+FC3D_API int fc3d_hip_batch_get_velocity(const fc3d_hip_batch* b, double* velocity);
+/* Solves every problem; per-problem info / sweeps / residual (may be NULL). */
+FC3D_API int fc3d_hip_batch_solve(fc3d_hip_batch* b, const fc3d_hip_params* p,
+                                  fc3d_hip_batch_stats* s, int* info, int* iterations,
+                                  double* residual);
 
 /* ---- colouring ----------------------------------------------------------- */
 /* Smallest colour not used by a lower-index neighbour, in index order;

@@ -80,7 +80,13 @@ class HipStats(C.Structure):
                 ("residual", C.c_double), ("incr_error", C.c_double), ("num_colours", C.c_int),
                 ("warm_start_used", C.c_int), ("solve_ms", C.c_double),
                 ("sweep_kernel_ms", C.c_double), ("sweep_kernel_launches", C.c_int64),
-                ("sweep_kernel_bytes", C.c_double)]
+                ("sweep_kernel_bytes", C.c_double), ("schedule", C.c_int)]
+
+
+class HipBatchStats(C.Structure):
+    _fields_ = [("solve_ms", C.c_double), ("converged", C.c_int),
+                ("converged_contacts", C.c_int64), ("total_sweeps", C.c_int64),
+                ("fp32_sweeps", C.c_int64), ("max_residual", C.c_double), ("schedule", C.c_int)]
 
 
 class Generated(C.Structure):
@@ -167,6 +173,15 @@ SIGNATURES = {
     "fc3d_hip_local_solve": (C.c_int, [C.c_int, c_double_p, c_double_p, c_double_p, c_double_p,
                                        C.POINTER(HipParams), c_double_p, C.c_int, c_int_p]),
     "fc3d_hip_colour": (C.c_int, [C.c_int, c_int64_p, c_int64_p, c_int_p]),
+    "fc3d_hip_batch_create": (C.c_int, [C.c_int, c_int_p, c_int64_p, c_int64_p, c_double_p,
+                                        c_double_p, c_double_p, C.c_int, C.POINTER(H_p)]),
+    "fc3d_hip_batch_destroy": (None, [H_p]),
+    "fc3d_hip_batch_order": (C.c_int, [H_p, c_int_p]),
+    "fc3d_hip_batch_set_reaction": (C.c_int, [H_p, c_double_p]),
+    "fc3d_hip_batch_get_reaction": (C.c_int, [H_p, c_double_p]),
+    "fc3d_hip_batch_get_velocity": (C.c_int, [H_p, c_double_p]),
+    "fc3d_hip_batch_solve": (C.c_int, [H_p, C.POINTER(HipParams), C.POINTER(HipBatchStats),
+                                       c_int_p, c_int_p, c_double_p]),
 }
 
 _lib = None

@@ -18,136 +18,33 @@
 // several sweeps replays without host round trips.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
-#include "fc3d_device.hpp"
 #include "fc3d_internal.hpp"
+#include "fc3d_layout.hpp"
+#include "fc3d_sweep.hpp"
 
 namespace fc3d {
 
 int colour_on_device(int nc, const int* d_rp, const int* d_ci, int* d_colour, hipStream_t s,
                      int* ncolours_out, std::vector<int>* host_colour);
 
-template <typename T> struct vec4;
-template <> struct vec4<double> { using type = double4; };
-template <> struct vec4<float> { using type = float4; };
-
-struct DevParams {
-  LocalParams lp;
-  double tol;           // user tolerance
-  double local_tol;     // local tolerance at solve start (FP64 phase)
-  double fp32_local_tol;
-  double fp32_tol;
-  double omega;
-  double rho_constant;
-  int mode;             // error evaluation (NonSmoothGaussSeidel_options.h:121-140)
-  int freq;             // error evaluation frequency
-  int max_iter;
-  int filter;
-  int relaxation;
-  int rho_strategy;
-};
-
-struct SolveState {
-  int iter;       // sweeps done (all phases)
-  int stop;       // 1: no further sweep
-  int notconv;    // reference hasNotConverged
-  int need_full;  // residual pending for this sweep
-  int freq;       // current evaluation frequency (ADAPTIVE doubles it)
-  int phase;      // 1: FP32 sweeps, 0: FP64 sweeps
-  int fp32_iter;
-  int pad;
-  double incr;
-  double full;
-  double working_tol;
-  double local_tol;
-  double norm_q;
-};
-
-template <typename T>
-struct SweepPtrs {
-  const int* len;   // off-diagonal blocks of the slot's row, -1 for padding
-  const int* dpos;  // position of the diagonal block in the reference row order
-  const int* koff;  // per slice: first k-plane
-  const int* kmax;  // per slice: number of k-planes
-  const int* col;   // [kplane][64] slot of the neighbour
-  const T* vals;    // [kplane][9][64]
-  const T* diag;    // [slice][9][64]
-  const T* q;       // [slice][3][64]
-  const T* mu;      // [slot]
-  typename vec4<T>::type* r;
-  typename vec4<T>::type* u;
-  T* pli;           // per-slot PLI step (persistent local state)
-  double* part;     // per-slice partial sums
-};
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
 // One colour of one sweep: fc3d_nsgs.c:737-822 for the contacts of slices
 // [s_begin, s_end), which share no body.
-template <typename T>
-__global__ __launch_bounds__(256) void k_sweep(SweepPtrs<T> P, const DevParams* __restrict__ prm,
-                                               const SolveState* __restrict__ st, int s_begin,
-                                               int s_end, int want_phase) {
+template <typename T, int U, int PF, int MINW>
+__global__ __launch_bounds__(256, MINW) void k_sweep(SweepPtrs<T> P,
+                                                     const DevParams* __restrict__ prm,
+                                                     const SolveState* __restrict__ st,
+                                                     int s_begin, int s_end, int want_phase) {
   if (st->stop || st->phase != want_phase) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int s = s_begin + blockIdx.x * 4 + wave;
+  const int s = __builtin_amdgcn_readfirstlane(s_begin + xcd_block() * 4 + wave);
   if (s >= s_end) return;
-  const int slot = s * 64 + lane;
-  const int len = P.len[slot];
-  double light = 0.0, nr2 = 0.0;
-  if (len >= 0) {
-    T ql[3];
-#pragma unroll
-    for (int e = 0; e < 3; e++) ql[e] = P.q[(size_t)(s * 3 + e) * 64 + lane];
-    const int koff = P.koff[s], kmax = P.kmax[s];
-    for (int k = 0; k < kmax; k++) {
-      if (k < len) {
-        const size_t plane = (size_t)(koff + k);
-        const int j = P.col[plane * 64 + lane];
-        T w[9];
-#pragma unroll
-        for (int e = 0; e < 9; e++) w[e] = P.vals[(plane * 9 + e) * 64 + lane];
-        const typename vec4<T>::type rj = P.r[j];
-        const T v[3] = {rj.x, rj.y, rj.z};
-        mvp3x3<T>(w, v, ql);
-      }
-    }
-    LocalProblem<T> lp;
-#pragma unroll
-    for (int e = 0; e < 9; e++) lp.W[e] = P.diag[(size_t)(s * 9 + e) * 64 + lane];
-    lp.q[0] = ql[0]; lp.q[1] = ql[1]; lp.q[2] = ql[2];
-    lp.mu = P.mu[slot];
-    const typename vec4<T>::type ro = P.r[slot];
-    const T rold[3] = {ro.x, ro.y, ro.z};
-    T lr[3] = {ro.x, ro.y, ro.z};
-    T rho[3];
-    compute_rho<T>(prm->rho_strategy, T(prm->rho_constant), lp.W, rho);
-    T pli = P.pli[slot];
-    T res = T(0);
-    const T tol = T(want_phase ? prm->fp32_local_tol : st->local_tol);
-    local_solve<T>(lp, lr, prm->lp, tol, &pli, rho, &res);
-    if (prm->relaxation) {
-      const T om = T(prm->omega);
-#pragma unroll
-      for (int e = 0; e < 3; e++) lr[e] = om * lr[e] + (T(1) - om) * rold[e];
-    }
-    const T d0 = rold[0] - lr[0], d1 = rold[1] - lr[1], d2 = rold[2] - lr[2];
-    light = double(d0 * d0 + d1 * d1 + d2 * d2);
-    const bool accept = !(prm->filter && (isnan(res) || isinf(res) || res > T(1)));
-    typename vec4<T>::type out = ro;
-    if (accept) { out.x = lr[0]; out.y = lr[1]; out.z = lr[2]; }
-    P.r[slot] = out;
-    P.pli[slot] = pli;
-    nr2 = double(out.x) * double(out.x) + double(out.y) * double(out.y) +
-          double(out.z) * double(out.z);
-    if (want_phase == 0) nr2 = double(out.x * out.x + out.y * out.y + out.z * out.z);
-  }
+  const T tol = T(want_phase ? prm->fp32_local_tol : st->local_tol);
+  double light, nr2;
+  sweep_slot<T, U, PF>(P, prm, s, lane, tol, &light, &nr2);
   light = wave_sum(light);
   nr2 = wave_sum(nr2);
   if (lane == 0) {
@@ -156,8 +53,8 @@ __global__ __launch_bounds__(256) void k_sweep(SweepPtrs<T> P, const DevParams* 
   }
 }
 
-// fc3d_compute_error.c:70-104 per slot: u = q + W r (blocks in the reference
-// row order, the diagonal at its own position), then the local term.
+// Residual pass (fc3d_compute_error.c:70-104) over all slices, or, with
+// gemv_only, y += W x on the resident layout (SparseBlockMatrix.c:360-419).
 __global__ __launch_bounds__(256) void k_residual(SweepPtrs<double> P, const SolveState* st,
                                                   int nslices, int force, const double* x_in,
                                                   int gemv_only) {
@@ -165,19 +62,13 @@ __global__ __launch_bounds__(256) void k_residual(SweepPtrs<double> P, const Sol
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int s = blockIdx.x * 4 + wave;
   if (s >= nslices) return;
-  const int slot = s * 64 + lane;
-  const int len = P.len[slot];
-  double e2 = 0.0, r2 = 0.0, u2 = 0.0;
-  if (len >= 0) {
-    double u[3];
-    if (gemv_only) {
-      const double4 y0 = P.u[slot];
-      u[0] = y0.x; u[1] = y0.y; u[2] = y0.z;
-    } else {
-#pragma unroll
-      for (int e = 0; e < 3; e++) u[e] = P.q[(size_t)(s * 3 + e) * 64 + lane];
-    }
-    const double4* X = gemv_only ? reinterpret_cast<const double4*>(x_in) : P.r;
+  if (gemv_only) {
+    const int slot = s * 64 + lane;
+    const int len = P.len[slot];
+    if (len < 0) return;
+    const double4 y0 = P.u[slot];
+    double u[3] = {y0.x, y0.y, y0.z};
+    const double4* X = reinterpret_cast<const double4*>(x_in);
     const double4 rs = X[slot];
     const double rv[3] = {rs.x, rs.y, rs.z};
     const int dpos = P.dpos[slot];
@@ -200,13 +91,10 @@ __global__ __launch_bounds__(256) void k_residual(SweepPtrs<double> P, const Sol
     }
     if (dpos >= kmax) mvp3x3<double>(dg, rv, u);
     P.u[slot] = make_double4(u[0], u[1], u[2], 0.0);
-    if (!gemv_only) {
-      e2 = unitary_error<double>(rv, u, P.mu[slot]);
-      r2 = rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
-      u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-    }
+    return;
   }
-  if (gemv_only) return;
+  double e2, r2, u2;
+  residual_slot(P, s, lane, &e2, &r2, &u2);
   e2 = wave_sum(e2);
   r2 = wave_sum(r2);
   u2 = wave_sum(u2);
@@ -384,6 +272,8 @@ struct fc3d_hip_problem {
   // captured sweep chunks: [phase] -> graph of `graph_sweeps` sweeps
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   int graph_sweeps[2] = {0, 0};
+  int graph_schedule[2] = {0, 0};
+  int tuned_schedule[2] = {0, 0};  // autotuner's choice per phase (0: not tuned)
 };
 
 namespace fc3d {
@@ -439,17 +329,40 @@ static int ensure_fp32(fc3d_hip_problem* h) {
 }
 
 // Enqueues one sweep of `phase` (1 FP32, 0 FP64) plus its convergence step.
-static void enqueue_sweep(fc3d_hip_problem* h, int phase, hipEvent_t* ev, int nev) {
+// Sweep-kernel schedules: unroll U, column prefetch PF, occupancy target.
+#define FC3D_NUM_SCHEDULES 6
+static const char* const kScheduleNames[FC3D_NUM_SCHEDULES + 1] = {
+    "auto", "u4/u8 occ2", "u4/u8 occ3", "u4/u8 occ4", "u4/u8 pf16 occ2", "u4/u8 pf16 occ3",
+    "u2/u4 pf8 occ4"};
+
+template <int U64, int U32, int PF, int MINW>
+static void launch_sweep_t(fc3d_hip_problem* h, int phase, int grid, int sb, int se) {
+  if (phase)
+    hipLaunchKernelGGL((k_sweep<float, U32, PF, MINW>), dim3(grid), dim3(256), 0, h->stream,
+                       ptrs32(h), h->prm, h->st, sb, se, 1);
+  else
+    hipLaunchKernelGGL((k_sweep<double, U64, PF, MINW>), dim3(grid), dim3(256), 0, h->stream,
+                       ptrs64(h, h->part), h->prm, h->st, sb, se, 0);
+}
+
+static void launch_sweep(fc3d_hip_problem* h, int phase, int schedule, int grid, int sb, int se) {
+  switch (schedule) {
+    case 2: launch_sweep_t<4, 8, 0, 3>(h, phase, grid, sb, se); break;
+    case 3: launch_sweep_t<4, 8, 0, 4>(h, phase, grid, sb, se); break;
+    case 4: launch_sweep_t<4, 8, 16, 2>(h, phase, grid, sb, se); break;
+    case 5: launch_sweep_t<4, 8, 16, 3>(h, phase, grid, sb, se); break;
+    case 6: launch_sweep_t<2, 4, 8, 4>(h, phase, grid, sb, se); break;
+    default: launch_sweep_t<4, 8, 0, 2>(h, phase, grid, sb, se); break;
+  }
+}
+
+static void enqueue_sweep(fc3d_hip_problem* h, int phase, int schedule, hipEvent_t* ev,
+                          int nev) {
   for (int c = 0; c < h->ncolours; c++) {
     const int sb = h->colour_slice_ptr[(size_t)c], se = h->colour_slice_ptr[(size_t)c + 1];
     const int grid = (se - sb + 3) / 4;
     if (ev && 2 * c + 1 < nev) hipEventRecord(ev[2 * c], h->stream);
-    if (phase)
-      hipLaunchKernelGGL(k_sweep<float>, dim3(grid), dim3(256), 0, h->stream, ptrs32(h), h->prm,
-                         h->st, sb, se, 1);
-    else
-      hipLaunchKernelGGL(k_sweep<double>, dim3(grid), dim3(256), 0, h->stream,
-                         ptrs64(h, h->part), h->prm, h->st, sb, se, 0);
+    launch_sweep(h, phase, schedule, grid, sb, se);
     if (ev && 2 * c + 1 < nev) hipEventRecord(ev[2 * c + 1], h->stream);
   }
   hipLaunchKernelGGL(k_finalize_sweep, dim3(1), dim3(1024), 0, h->stream, h->prm, h->st, h->part,
@@ -462,19 +375,21 @@ static void enqueue_sweep(fc3d_hip_problem* h, int phase, hipEvent_t* ev, int ne
   }
 }
 
-static int build_graph(fc3d_hip_problem* h, int phase, int sweeps) {
-  if (h->graph[phase] && h->graph_sweeps[phase] == sweeps) return 0;
+static int build_graph(fc3d_hip_problem* h, int phase, int sweeps, int schedule) {
+  if (h->graph[phase] && h->graph_sweeps[phase] == sweeps && h->graph_schedule[phase] == schedule)
+    return 0;
   if (h->graph[phase]) {
     hipGraphExecDestroy(h->graph[phase]);
     h->graph[phase] = nullptr;
   }
   hipGraph_t g = nullptr;
   FC3D_HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < sweeps; k++) enqueue_sweep(h, phase, nullptr, 0);
+  for (int k = 0; k < sweeps; k++) enqueue_sweep(h, phase, schedule, nullptr, 0);
   FC3D_HIP_TRY(hipStreamEndCapture(h->stream, &g));
   FC3D_HIP_TRY(hipGraphInstantiate(&h->graph[phase], g, nullptr, nullptr, 0));
   hipGraphDestroy(g);
   h->graph_sweeps[phase] = sweeps;
+  h->graph_schedule[phase] = schedule;
   return 0;
 }
 
@@ -484,6 +399,69 @@ static int final_residual(fc3d_hip_problem* h) {
   hipLaunchKernelGGL(k_finalize_residual, dim3(1), dim3(1024), 0, h->stream, h->prm, h->st,
                      h->part2, h->nslices, 1);
   FC3D_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// Autotuner: times every schedule on two sweeps of this very problem (from
+// the resident reaction, restored afterwards) and keeps the fastest.  The
+// choice is cached on the handle and, per size class, for the process.
+struct TuneKey {
+  int log2_slices, colours, phase;
+};
+static std::vector<std::pair<TuneKey, int>> g_tune_cache;
+
+static int tune_schedule(fc3d_hip_problem* h, int phase, double norm_q, double local_tol,
+                         int* best) {
+  int lg = 0;
+  while ((1 << (lg + 1)) <= h->nslices) lg++;
+  for (auto& e : g_tune_cache)
+    if (e.first.log2_slices == lg && e.first.colours == h->ncolours && e.first.phase == phase) {
+      *best = e.second;
+      return 0;
+    }
+  const size_t ns = (size_t)h->nslots;
+  void* saved = phase ? (void*)h->u : (void*)h->x;  // scratch of at least ns * 16 / 32 bytes
+  void* live = phase ? (void*)h->r32 : (void*)h->r;
+  const size_t bytes = ns * (phase ? sizeof(float4) : sizeof(double4));
+  FC3D_HIP_TRY(hipMemcpyAsync(saved, live, bytes, hipMemcpyDeviceToDevice, h->stream));
+  std::vector<hipEvent_t> ev(2 * (size_t)h->ncolours);
+  for (auto& e : ev) FC3D_HIP_TRY(hipEventCreate(&e));
+  SolveState t;
+  std::memset(&t, 0, sizeof t);
+  t.iter = -1000000;  // never reaches max_iter while tuning
+  t.notconv = 1;
+  t.phase = phase;
+  t.incr = INFINITY;
+  t.full = INFINITY;
+  t.working_tol = 0.0;  // never converges while tuning
+  t.local_tol = local_tol;
+  t.norm_q = norm_q;
+  double best_ms = 1e300;
+  *best = 1;
+  for (int sc = 1; sc <= FC3D_NUM_SCHEDULES; sc++) {
+    FC3D_HIP_TRY(hipMemcpyAsync(h->st, &t, sizeof t, hipMemcpyHostToDevice, h->stream));
+    FC3D_HIP_TRY(hipMemcpyAsync(live, saved, bytes, hipMemcpyDeviceToDevice, h->stream));
+    double ms_total = 0.0;
+    for (int rep = 0; rep < 3; rep++) {
+      enqueue_sweep(h, phase, sc, ev.data(), (int)ev.size());
+      FC3D_HIP_TRY(hipGetLastError());
+      FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
+      if (rep == 0) continue;  // first sweep warms caches and code
+      for (int c = 0; c < h->ncolours; c++) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, ev[2 * (size_t)c], ev[2 * (size_t)c + 1]);
+        ms_total += ms;
+      }
+    }
+    if (ms_total < best_ms) {
+      best_ms = ms_total;
+      *best = sc;
+    }
+  }
+  for (auto& e : ev) hipEventDestroy(e);
+  FC3D_HIP_TRY(hipMemcpyAsync(live, saved, bytes, hipMemcpyDeviceToDevice, h->stream));
+  FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
+  g_tune_cache.push_back({{lg, h->ncolours, phase}, *best});
   return 0;
 }
 
@@ -529,36 +507,11 @@ int fc3d_hip_problem_create(int nc, const int64_t* row_ptr, const int64_t* col_i
     set_error("fc3d_hip_problem_create: row_ptr[0] must be 0 and blocks < 2^31");
     return NUMERICS_ERR_INVALID_ARGUMENT;
   }
-  // validate the structure and find each row's diagonal block
-  std::vector<int> offlen((size_t)nc), dpos_row((size_t)nc);
-  std::vector<int64_t> diag_block((size_t)nc);
-  for (int i = 0; i < nc; i++) {
-    if (row_ptr[i + 1] < row_ptr[i]) {
-      set_error("fc3d_hip_problem_create: row pointers not monotone at row %d", i);
-      return NUMERICS_ERR_INVALID_ARGUMENT;
-    }
-    int64_t d = -1;
-    for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
-      if (col_idx[k] < 0 || col_idx[k] >= nc) {
-        set_error("fc3d_hip_problem_create: column %lld out of range in row %d",
-                  (long long)col_idx[k], i);
-        return NUMERICS_ERR_INVALID_ARGUMENT;
-      }
-      if (col_idx[k] == i) {
-        if (d >= 0) {
-          set_error("fc3d_hip_problem_create: two diagonal blocks in row %d", i);
-          return NUMERICS_ERR_INVALID_ARGUMENT;
-        }
-        d = k;
-      }
-    }
-    if (d < 0) {
-      set_error("fc3d_hip_problem_create: row %d has no diagonal block", i);
-      return NUMERICS_ERR_INVALID_ARGUMENT;
-    }
-    diag_block[(size_t)i] = d;
-    dpos_row[(size_t)i] = (int)(d - row_ptr[i]);
-    offlen[(size_t)i] = (int)(row_ptr[i + 1] - row_ptr[i] - 1);
+  RowInfo info((size_t)nc);
+  std::string err;
+  if (int e = validate_rows(0, nc, row_ptr, col_idx, &info, &err)) {
+    set_error("fc3d_hip_problem_create: %s", err.c_str());
+    return e;
   }
   int rc = use_device(device);
   if (rc) return rc;
@@ -598,81 +551,27 @@ int fc3d_hip_problem_create(int nc, const int64_t* row_ptr, const int64_t* col_i
     if (rc) return fail(rc);
   }
 
-  // colour-major order; inside a colour, longest rows first (ties by index)
-  std::vector<std::vector<int>> members((size_t)h->ncolours);
-  for (int i = 0; i < nc; i++) members[(size_t)h->colour[(size_t)i]].push_back(i);
-  h->colour_slice_ptr.assign((size_t)h->ncolours + 1, 0);
-  for (int c = 0; c < h->ncolours; c++) {
-    auto& m = members[(size_t)c];
-    std::stable_sort(m.begin(), m.end(), [&](int a, int b) {
-      return offlen[(size_t)a] > offlen[(size_t)b];
-    });
-    h->colour_slice_ptr[(size_t)c + 1] =
-        h->colour_slice_ptr[(size_t)c] + (int)((m.size() + 63) / 64);
-  }
-  h->nslices = h->colour_slice_ptr[(size_t)h->ncolours];
-  h->nslots = h->nslices * 64;
-  h->slot2orig.assign((size_t)h->nslots, -1);
-  h->orig2slot.assign((size_t)nc, -1);
-  h->colour_bytes64.assign((size_t)h->ncolours, 0.0);
-  h->colour_bytes32.assign((size_t)h->ncolours, 0.0);
-  for (int c = 0; c < h->ncolours; c++) {
-    const auto& m = members[(size_t)c];
-    for (size_t t = 0; t < m.size(); t++) {
-      const int slot = h->colour_slice_ptr[(size_t)c] * 64 + (int)t;
-      h->slot2orig[(size_t)slot] = m[t];
-      h->orig2slot[(size_t)m[t]] = slot;
-      const double kbar = offlen[(size_t)m[t]] + 1.0;
-      h->colour_bytes64[(size_t)c] += 76.0 * kbar + 108.0;
-      h->colour_bytes32[(size_t)c] += 40.0 * kbar + 56.0;
-    }
-  }
-  std::vector<int> len((size_t)h->nslots, -1), dpos((size_t)h->nslots, 0);
-  std::vector<int> koff((size_t)h->nslices), kmax((size_t)h->nslices);
-  int64_t planes = 0;
-  for (int sl = 0; sl < h->nslices; sl++) {
-    int mx = 0;
-    for (int l = 0; l < 64; l++) {
-      const int o = h->slot2orig[(size_t)sl * 64 + l];
-      if (o >= 0) mx = std::max(mx, offlen[(size_t)o]);
-    }
-    koff[(size_t)sl] = (int)planes;
-    kmax[(size_t)sl] = mx;
-    planes += mx;
-  }
-  if (planes * 64 > 0x7fffffffLL) {
+  HostLayout L;
+  int ncol = 0;
+  append_problem(L, 0, nc, row_ptr, col_idx, blocks, q, mu, h->colour.data(), info,
+                 default_sort_window(), &ncol);
+  if (L.nplanes * 64 > 0x7fffffffLL) {
     set_error("fc3d_hip_problem_create: sliced layout exceeds int32 indexing");
     return fail(NUMERICS_ERR_INVALID_ARGUMENT);
   }
-  h->nplanes = planes;
-  std::vector<int> colv((size_t)(planes > 0 ? planes : 1) * 64, 0);
-  std::vector<double> vals((size_t)(planes > 0 ? planes : 1) * 9 * 64, 0.0);
-  std::vector<double> diag((size_t)h->nslices * 9 * 64, 0.0), qv((size_t)h->nslices * 3 * 64, 0.0);
-  std::vector<double> muv((size_t)h->nslots, 0.0);
-  for (int slot = 0; slot < h->nslots; slot++) {
-    const int i = h->slot2orig[(size_t)slot];
-    if (i < 0) continue;
-    const int sl = slot / 64, l = slot % 64;
-    len[(size_t)slot] = offlen[(size_t)i];
-    dpos[(size_t)slot] = dpos_row[(size_t)i];
-    int kk = 0;
-    for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
-      if (k == diag_block[(size_t)i]) continue;
-      const size_t plane = (size_t)koff[(size_t)sl] + (size_t)kk;
-      colv[plane * 64 + (size_t)l] = h->orig2slot[(size_t)col_idx[k]];
-      for (int e = 0; e < 9; e++) vals[(plane * 9 + (size_t)e) * 64 + (size_t)l] = blocks[9 * k + e];
-      kk++;
-    }
-    const int64_t d = diag_block[(size_t)i];
-    for (int e = 0; e < 9; e++) diag[((size_t)sl * 9 + (size_t)e) * 64 + (size_t)l] = blocks[9 * d + e];
-    for (int e = 0; e < 3; e++) qv[((size_t)sl * 3 + (size_t)e) * 64 + (size_t)l] = q[3 * i + e];
-    muv[(size_t)slot] = mu[i];
-  }
-  if ((rc = dev_upload(&h->len, len)) || (rc = dev_upload(&h->dpos, dpos)) ||
-      (rc = dev_upload(&h->koff, koff)) || (rc = dev_upload(&h->kmax, kmax)) ||
-      (rc = dev_upload(&h->col, colv)) || (rc = dev_upload(&h->vals, vals)) ||
-      (rc = dev_upload(&h->diag, diag)) || (rc = dev_upload(&h->q, qv)) ||
-      (rc = dev_upload(&h->mu, muv)) || (rc = dev_alloc(&h->r, (size_t)h->nslots)) ||
+  h->nslices = L.nslices;
+  h->nslots = L.nslots;
+  h->nplanes = L.nplanes;
+  h->colour_slice_ptr = L.colour_slice_ptr;
+  h->slot2orig = L.slot2orig;
+  h->orig2slot = L.orig2slot;
+  h->colour_bytes64 = L.colour_bytes64;
+  h->colour_bytes32 = L.colour_bytes32;
+  if ((rc = dev_upload(&h->len, L.len)) || (rc = dev_upload(&h->dpos, L.dpos)) ||
+      (rc = dev_upload(&h->koff, L.koff)) || (rc = dev_upload(&h->kmax, L.kmax)) ||
+      (rc = dev_upload(&h->col, L.col)) || (rc = dev_upload(&h->vals, L.vals)) ||
+      (rc = dev_upload(&h->diag, L.diag)) || (rc = dev_upload(&h->q, L.q)) ||
+      (rc = dev_upload(&h->mu, L.mu)) || (rc = dev_alloc(&h->r, (size_t)h->nslots)) ||
       (rc = dev_alloc(&h->u, (size_t)h->nslots)) || (rc = dev_alloc(&h->x, (size_t)h->nslots)) ||
       (rc = dev_alloc(&h->pli, (size_t)h->nslots)) ||
       (rc = dev_alloc(&h->part, 2 * (size_t)h->nslices)) ||
@@ -827,6 +726,27 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
   const int mixed = p->precision == 1;
   if (mixed && (rc = ensure_fp32(h))) return rc;
 
+  int sched[2] = {p->schedule, p->schedule};
+  if (p->schedule < 0 || p->schedule > FC3D_NUM_SCHEDULES) {
+    set_error("fc3d_hip_solve: schedule %d out of range [0, %d]", p->schedule, FC3D_NUM_SCHEDULES);
+    return NUMERICS_ERR_INVALID_ARGUMENT;
+  }
+  if (p->schedule == 0) {
+    for (int ph = 0; ph <= mixed; ph++) {
+      if (!h->tuned_schedule[ph]) {
+        if (ph == 1)
+          hipLaunchKernelGGL(k_r64_to_r32, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0,
+                             h->stream, h->r, h->r32, h->pli, h->pli32, ns);
+        if ((rc = tune_schedule(h, ph, h->norm_q, p->local_tol, &h->tuned_schedule[ph])))
+          return rc;
+        hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0,
+                           h->stream, h->pli, 1.0, ns);
+      }
+      sched[ph] = h->tuned_schedule[ph];
+    }
+  }
+  s->schedule = sched[mixed];
+
   SolveState s0;
   std::memset(&s0, 0, sizeof s0);
   s0.stop = p->max_iter <= 0 ? 1 : 0;
@@ -850,6 +770,10 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
   }
   FC3D_HIP_TRY(hipEventRecord(t0, h->stream));
   const int chunk = p->sweeps_per_launch > 0 ? p->sweeps_per_launch : 1;
+  // FC3D_HIP_NO_GRAPH=1 enqueues the same kernels directly instead of
+  // replaying a captured graph (for tools that do not follow graph launches)
+  const char* ng = getenv("FC3D_HIP_NO_GRAPH");
+  const int no_graph = ng && ng[0] == '1';
 
   for (int phase = mixed ? 1 : 0; phase >= 0; phase--) {
     if (phase == 0 && mixed) {
@@ -877,7 +801,7 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
         return NUMERICS_ERR_UNKNOWN;
       }
       if (p->instrument) {
-        enqueue_sweep(h, phase, kev.data(), (int)kev.size());
+        enqueue_sweep(h, phase, sched[phase], kev.data(), (int)kev.size());
         FC3D_HIP_TRY(hipGetLastError());
         FC3D_HIP_TRY(hipMemcpyAsync(&cur, h->st, sizeof cur, hipMemcpyDeviceToHost, h->stream));
         FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
@@ -891,8 +815,13 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
                 phase ? h->colour_bytes32[(size_t)c] : h->colour_bytes64[(size_t)c];
           }
         }
+      } else if (no_graph) {
+        for (int k = 0; k < chunk; k++) enqueue_sweep(h, phase, sched[phase], nullptr, 0);
+        FC3D_HIP_TRY(hipGetLastError());
+        FC3D_HIP_TRY(hipMemcpyAsync(&cur, h->st, sizeof cur, hipMemcpyDeviceToHost, h->stream));
+        FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
       } else {
-        if ((rc = build_graph(h, phase, chunk))) return rc;
+        if ((rc = build_graph(h, phase, chunk, sched[phase]))) return rc;
         FC3D_HIP_TRY(hipGraphLaunch(h->graph[phase], h->stream));
         FC3D_HIP_TRY(hipMemcpyAsync(&cur, h->st, sizeof cur, hipMemcpyDeviceToHost, h->stream));
         FC3D_HIP_TRY(hipStreamSynchronize(h->stream));

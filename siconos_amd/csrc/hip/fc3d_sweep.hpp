@@ -1,0 +1,235 @@
+// fc3d_sweep.hpp — device-side pieces shared by the single-problem solver
+// (fc3d_problem.hip) and the batched solver (fc3d_batch.hip): parameter and
+// state blocks, the sliced-ELL access, and the per-slot sweep and residual.
+#pragma once
+#include "fc3d_device.hpp"
+
+namespace fc3d {
+
+template <typename T> struct vec4;
+template <> struct vec4<double> { using type = double4; };
+template <> struct vec4<float> { using type = float4; };
+
+struct DevParams {
+  LocalParams lp;
+  double tol;           // user tolerance
+  double local_tol;     // local tolerance at solve start (FP64 phase)
+  double fp32_local_tol;
+  double fp32_tol;
+  double omega;
+  double rho_constant;
+  int mode;             // error evaluation (NonSmoothGaussSeidel_options.h:121-140)
+  int freq;             // error evaluation frequency
+  int max_iter;
+  int filter;
+  int relaxation;
+  int rho_strategy;
+};
+
+struct SolveState {
+  int iter;       // sweeps done (all phases)
+  int stop;       // 1: no further sweep
+  int notconv;    // reference hasNotConverged
+  int need_full;  // residual pending for this sweep
+  int freq;       // current evaluation frequency (ADAPTIVE doubles it)
+  int phase;      // 1: FP32 sweeps, 0: FP64 sweeps
+  int fp32_iter;
+  int pad;
+  double incr;
+  double full;
+  double working_tol;
+  double local_tol;
+  double norm_q;
+};
+
+template <typename T>
+struct SweepPtrs {
+  const int* len;   // off-diagonal blocks of the slot's row, -1 for padding
+  const int* dpos;  // position of the diagonal block in the reference row order
+  const int* koff;  // per slice: first k-plane
+  const int* kmax;  // per slice: number of k-planes
+  const int* col;   // [kplane][64] slot of the neighbour
+  const T* vals;    // [kplane][9][64]
+  const T* diag;    // [slice][9][64]
+  const T* q;       // [slice][3][64]
+  const T* mu;      // [slot]
+  typename vec4<T>::type* r;
+  typename vec4<T>::type* u;
+  T* pli;           // per-slot PLI step (persistent local state)
+  double* part;     // per-slice partial sums
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// y += sum_k W_k r_{col(k)} over the off-diagonal blocks of the lane's row, in
+// the reference row order (SBM_row_prod_no_diag_3x3, SparseBlockMatrix.c:
+// 1431-1490).  The slice index is wave-uniform, so its metadata are scalar
+// loads; planes are read U at a time (padded entries hold a zero block and a
+// valid column) so that U blocks and U neighbour reactions are in flight
+// together; only blocks k < len enter the sum.
+template <typename T, int U, int PF>
+__device__ __forceinline__ void gather_row(const SweepPtrs<T>& P, int s, int lane, int len,
+                                           T* y) {
+  const int koff = __builtin_amdgcn_readfirstlane(P.koff[s]);
+  const int kmax = __builtin_amdgcn_readfirstlane(P.kmax[s]);
+  const int* __restrict__ colp = P.col + (size_t)koff * 64 + lane;
+  const T* __restrict__ valp = P.vals + (size_t)koff * 9 * 64 + lane;
+  int k0 = 0;
+  if (PF > 0) {
+    // the first PF column indices are loaded at once, so that only one
+    // memory latency separates them from the neighbour reactions
+    int jj[PF > 0 ? PF : 1];
+#pragma unroll
+    for (int u = 0; u < PF; u++)
+      if (u < kmax) jj[u] = colp[(size_t)u * 64];
+#pragma unroll
+    for (int b = 0; b < PF; b += U) {
+      if (b >= kmax) break;
+      T w[U][9];
+      typename vec4<T>::type rj[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (b + u < kmax) {
+          rj[u] = P.r[jj[b + u]];
+#pragma unroll
+          for (int e = 0; e < 9; e++) w[u][e] = valp[((size_t)(b + u) * 9 + e) * 64];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (b + u < len) {
+          const T v[3] = {rj[u].x, rj[u].y, rj[u].z};
+          mvp3x3<T>(w[u], v, y);
+        }
+      }
+    }
+    k0 = PF;
+  }
+  for (; k0 < kmax; k0 += U) {
+    int j[U];
+    T w[U][9];
+    typename vec4<T>::type rj[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (k0 + u < kmax) {
+        j[u] = colp[(size_t)(k0 + u) * 64];
+#pragma unroll
+        for (int e = 0; e < 9; e++) w[u][e] = valp[((size_t)(k0 + u) * 9 + e) * 64];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (k0 + u < kmax) rj[u] = P.r[j[u]];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (k0 + u < len) {
+        const T v[3] = {rj[u].x, rj[u].y, rj[u].z};
+        mvp3x3<T>(w[u], v, y);
+      }
+    }
+  }
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (block b on XCD
+// b % 8).  This maps the blocks of each XCD onto one contiguous range of
+// slices, so that the neighbour gathers of an XCD stay within its own L2.
+__device__ __forceinline__ int xcd_block() {
+  const int b = blockIdx.x, nb = gridDim.x;
+  const int x = b & 7, q = nb >> 3, rem = nb & 7;
+  return x * q + (x < rem ? x : rem) + (b >> 3);
+}
+
+// The per-contact step of a sweep (fc3d_nsgs.c:737-822) for the lane's slot of
+// slice s: local problem (fc3d_nsgs_update), local solve, relaxation, light
+// error term, optional filter, write-back.  Returns |r_old - r_new|^2 and the
+// squared norm of the accepted reaction (0 for padding slots).
+template <typename T, int U, int PF>
+__device__ __forceinline__ void sweep_slot(const SweepPtrs<T>& P, const DevParams* __restrict__ prm,
+                                           int s, int lane, T tol, double* light, double* nr2) {
+  *light = 0.0;
+  *nr2 = 0.0;
+  const int slot = s * 64 + lane;
+  const int len = P.len[slot];
+  if (len >= 0) {
+    // local-solve inputs first: their loads overlap the gather
+    LocalProblem<T> lp;
+#pragma unroll
+    for (int e = 0; e < 9; e++) lp.W[e] = P.diag[(size_t)(s * 9 + e) * 64 + lane];
+    lp.mu = P.mu[slot];
+    const typename vec4<T>::type ro = P.r[slot];
+    T pli = P.pli[slot];
+    T ql[3];
+#pragma unroll
+    for (int e = 0; e < 3; e++) ql[e] = P.q[(size_t)(s * 3 + e) * 64 + lane];
+    gather_row<T, U, PF>(P, s, lane, len, ql);
+    lp.q[0] = ql[0]; lp.q[1] = ql[1]; lp.q[2] = ql[2];
+    const T rold[3] = {ro.x, ro.y, ro.z};
+    T lr[3] = {ro.x, ro.y, ro.z};
+    T rho[3];
+    compute_rho<T>(prm->rho_strategy, T(prm->rho_constant), lp.W, rho);
+    T res = T(0);
+    local_solve<T>(lp, lr, prm->lp, tol, &pli, rho, &res);
+    if (prm->relaxation) {
+      const T om = T(prm->omega);
+#pragma unroll
+      for (int e = 0; e < 3; e++) lr[e] = om * lr[e] + (T(1) - om) * rold[e];
+    }
+    const T d0 = rold[0] - lr[0], d1 = rold[1] - lr[1], d2 = rold[2] - lr[2];
+    *light = double(d0 * d0 + d1 * d1 + d2 * d2);
+    const bool accept = !(prm->filter && (isnan(res) || isinf(res) || res > T(1)));
+    typename vec4<T>::type out = ro;
+    if (accept) { out.x = lr[0]; out.y = lr[1]; out.z = lr[2]; }
+    P.r[slot] = out;
+    P.pli[slot] = pli;
+    *nr2 = sizeof(T) == 8 ? double(out.x * out.x + out.y * out.y + out.z * out.z)
+                          : double(out.x) * double(out.x) + double(out.y) * double(out.y) +
+                                double(out.z) * double(out.z);
+  }
+}
+
+// fc3d_compute_error.c:70-104 for the lane's slot: u = q + W r (blocks in the
+// reference row order, the diagonal at its own position), u stored, and the
+// local error term with the squared norms of r and u returned.
+__device__ __forceinline__ void residual_slot(const SweepPtrs<double>& P, int s, int lane,
+                                              double* e2, double* r2, double* u2) {
+  const int slot = s * 64 + lane;
+  const int len = P.len[slot];
+  *e2 = 0.0;
+  *r2 = 0.0;
+  *u2 = 0.0;
+  if (len < 0) return;
+  double u[3];
+#pragma unroll
+  for (int e = 0; e < 3; e++) u[e] = P.q[(size_t)(s * 3 + e) * 64 + lane];
+  const double4 rs = P.r[slot];
+  const double rv[3] = {rs.x, rs.y, rs.z};
+  const int dpos = P.dpos[slot];
+  const int koff = P.koff[s], kmax = P.kmax[s];
+  double dg[9];
+#pragma unroll
+  for (int e = 0; e < 9; e++) dg[e] = P.diag[(size_t)(s * 9 + e) * 64 + lane];
+  for (int k = 0; k < kmax; k++) {
+    if (k == dpos) mvp3x3<double>(dg, rv, u);
+    if (k < len) {
+      const size_t plane = (size_t)(koff + k);
+      const int j = P.col[plane * 64 + lane];
+      double w[9];
+#pragma unroll
+      for (int e = 0; e < 9; e++) w[e] = P.vals[(plane * 9 + e) * 64 + lane];
+      const double4 rj = P.r[j];
+      const double v[3] = {rj.x, rj.y, rj.z};
+      mvp3x3<double>(w, v, u);
+    }
+  }
+  if (dpos >= kmax) mvp3x3<double>(dg, rv, u);
+  P.u[slot] = make_double4(u[0], u[1], u[2], 0.0);
+  *e2 = unitary_error<double>(rv, u, P.mu[slot]);
+  *r2 = rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
+  *u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+}
+
+}  // namespace fc3d

@@ -352,3 +352,89 @@ class HipProblem:
         if h:
             self._lib.fc3d_hip_problem_destroy(h)
             self._h = None
+
+
+class Batch:
+    """Many independent problems, solved in one persistent launch
+    (fc3d_hip_batch_*).  ``problems`` is a list of (SparseBlock3, q, mu)."""
+
+    def __init__(self, problems, device=0):
+        self._lib = capi.load()
+        self.sizes = np.array([W.nc for W, _, _ in problems], dtype=np.int32)
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        rps, cis, vals, qs, mus = [0], [], [], [], []
+        base_b = 0
+        for (W, q, mu), c0 in zip(problems, self.offsets[:-1]):
+            rps.append(W.row_ptr[1:] + base_b)
+            cis.append(W.col_idx + c0)
+            vals.append(W.values)
+            qs.append(np.asarray(q, dtype=np.float64))
+            mus.append(np.asarray(mu, dtype=np.float64))
+            base_b += W.nnzb
+        self.row_ptr = np.concatenate([np.array([0], dtype=np.int64)] + [np.asarray(x, dtype=np.int64) for x in rps[1:]])
+        self.col_idx = np.ascontiguousarray(np.concatenate(cis), dtype=np.int64)
+        self.values = np.ascontiguousarray(np.concatenate(vals))
+        self.q = np.ascontiguousarray(np.concatenate(qs))
+        self.mu = np.ascontiguousarray(np.concatenate(mus))
+        self.ntotal = int(self.offsets[-1])
+        h = C.c_void_p()
+        rc = self._lib.fc3d_hip_batch_create(len(problems), iptr(self.sizes), lptr(self.row_ptr),
+                                             lptr(self.col_idx), dptr(self.values), dptr(self.q),
+                                             dptr(self.mu), device, C.byref(h))
+        if rc:
+            raise RuntimeError(f"fc3d_hip_batch_create: {rc} {capi.last_error()}")
+        self._h = h
+
+    def _check(self, rc, what):
+        if rc:
+            raise RuntimeError(f"{what}: {rc} {capi.last_error()}")
+
+    def order(self):
+        o = np.empty(self.ntotal, dtype=np.int32)
+        self._check(self._lib.fc3d_hip_batch_order(self._h, iptr(o)), "order")
+        return o
+
+    def set_reaction(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        self._check(self._lib.fc3d_hip_batch_set_reaction(self._h, dptr(r)), "set_reaction")
+
+    def reaction(self):
+        r = np.empty(3 * self.ntotal)
+        self._check(self._lib.fc3d_hip_batch_get_reaction(self._h, dptr(r)), "get_reaction")
+        return r
+
+    def velocity(self):
+        u = np.empty(3 * self.ntotal)
+        self._check(self._lib.fc3d_hip_batch_get_velocity(self._h, dptr(u)), "get_velocity")
+        return u
+
+    def solve(self, params):
+        n = len(self.sizes)
+        s = capi.HipBatchStats()
+        info = np.empty(n, dtype=np.int32)
+        iters = np.empty(n, dtype=np.int32)
+        res = np.empty(n)
+        self._check(self._lib.fc3d_hip_batch_solve(self._h, C.byref(params), C.byref(s),
+                                                   iptr(info), iptr(iters), dptr(res)), "solve")
+        return s, info, iters, res
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.fc3d_hip_batch_destroy(h)
+            self._h = None
+
+
+def generate_batch(nprob, nc_lo, nc_hi, mu_lo, mu_hi, seed, bodies_per_contact=0.4):
+    """Independent random-pair problems with nc ~ U[nc_lo, nc_hi] and
+    bodies = nc / 2.5 (seeded)."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(nc_lo, nc_hi + 1, size=nprob)
+    out = []
+    for k, n in enumerate(sizes):
+        nb = max(3, int(round(n * bodies_per_contact)))
+        while n > nb * (nb - 1) // 2:
+            nb += 1
+        g = generate_random_pairs(nb, int(n), mu_lo, mu_hi, seed * 1000003 + k)
+        out.append((g.W, g.q, g.mu))
+    return out

@@ -1,0 +1,78 @@
+"""Many independent problems in one persistent launch (fc3d_hip_batch_*),
+each checked against the oracle run in the same per-problem sweep order."""
+import numpy as np
+import pytest
+
+from conftest import natural_residual
+from siconos_amd import numerics as N
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def batch():
+    probs = N.generate_batch(96, 8, 512, 0.1, 0.9, 11)
+    return probs, N.Batch(probs)
+
+
+def per_problem_order(B, order, k, nc):
+    c0 = B.offsets[k]
+    sel = order[(order >= c0) & (order < c0 + nc)]
+    return sel - c0
+
+
+@pytest.mark.parametrize("schedule", [1, 2])
+def test_batch_fp64_bit_exact_per_problem(oracle, batch, schedule):
+    probs, B = batch
+    B.set_reaction(np.zeros(3 * B.ntotal))
+    s, info, iters, res = B.solve(N.hip_params(tol=1e-6, max_iter=1000, schedule=schedule))
+    assert s.schedule == schedule
+    r, u, order = B.reaction(), B.velocity(), B.order()
+    for k, (W, q, mu) in enumerate(probs):
+        c0 = B.offsets[k]
+        io, ro, uo, it, reso = oracle.nsgs(W, q, mu, tol=1e-6, max_iter=1000,
+                                           order=per_problem_order(B, order, k, W.nc))
+        assert info[k] == io and iters[k] == it, k
+        np.testing.assert_array_equal(r[3 * c0:3 * (c0 + W.nc)], ro, err_msg=f"problem {k}")
+        np.testing.assert_array_equal(u[3 * c0:3 * (c0 + W.nc)], uo, err_msg=f"problem {k}")
+        np.testing.assert_allclose(res[k], reso, rtol=1e-12)
+    assert s.converged == int(np.sum(res <= 1e-6))
+
+
+def test_batch_mixed_precision_meets_tolerance(batch):
+    probs, B = batch
+    B.set_reaction(np.zeros(3 * B.ntotal))
+    s, info, iters, res = B.solve(N.hip_params(tol=1e-6, max_iter=1000, precision=1))
+    assert s.fp32_sweeps > 0
+    r = B.reaction()
+    for k, (W, q, mu) in enumerate(probs):
+        if info[k] != 0:
+            continue
+        c0 = B.offsets[k]
+        e, _ = natural_residual(W, q, mu, r[3 * c0:3 * (c0 + W.nc)])
+        assert e <= 1e-6 * (1 + 1e-9), k
+    assert s.converged >= 0.95 * len(probs)
+
+
+def test_batch_autotuner_and_warm_start(batch):
+    probs, B = batch
+    B.set_reaction(np.zeros(3 * B.ntotal))
+    s1, info1, it1, res1 = B.solve(N.hip_params(tol=1e-6, max_iter=1000, schedule=0))
+    assert s1.schedule in (1, 2)
+    r1 = B.reaction()
+    # a converged start stays converged: one sweep at most per problem
+    s2, info2, it2, res2 = B.solve(N.hip_params(tol=1e-6, max_iter=1000, schedule=0))
+    assert np.all(it2[info1 == 0] <= 2)
+    # warm_start = 1 restarts from zero and reproduces the first solve
+    s3, info3, it3, res3 = B.solve(N.hip_params(tol=1e-6, max_iter=1000, warm_start=1))
+    np.testing.assert_array_equal(B.reaction(), r1)
+
+
+def test_batch_rejects_cross_problem_columns():
+    probs = N.generate_batch(3, 8, 20, 0.1, 0.9, 12)
+    W, q, mu = probs[1]
+    ci = W.col_idx.copy()
+    ci[0] = W.nc + 3  # points into the next problem
+    bad = [probs[0], (N.SparseBlock3(W.row_ptr, ci, W.values), q, mu), probs[2]]
+    with pytest.raises(RuntimeError, match="out of range"):
+        N.Batch(bad)
