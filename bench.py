@@ -33,6 +33,17 @@ WORKLOADS = {
 }
 
 
+# HBM traffic per FP64 sweep-kernel launch, from two rocprofv3 --pmc passes
+# (FETCH_SIZE, WRITE_SIZE) over `bench.py --steps 1 --warmup 0 --no-cpu`,
+# scaled by the readings of a plain 1 GiB device copy (FETCH x2.000,
+# WRITE x1.000).  A profiler pass cannot run inside the timed bench, so the
+# figure is carried here with its source.
+PMC_TRAFFIC = {
+    "pile1m": {"bytes_per_launch": 100.49e6, "fetch": 97.80e6, "write": 2.69e6,
+               "source": "profiles/k_sweep_pile1m_hbm_traffic.md"},
+}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -78,6 +89,19 @@ def cpu_baseline(g, sweeps_needed, sample_sweeps):
                        f"{os.cpu_count()} logical cpus")}
 
 
+def copy_calibration(mib):
+    """A plain device-to-device copy of known size: its per-dispatch
+    FETCH_SIZE and WRITE_SIZE readings convert the counters to bytes."""
+    import torch
+    n = mib << 20
+    x = torch.ones(n, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,11 +110,18 @@ def main():
     ap.add_argument("--workload", default="pile1m", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-sweeps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 5 times (scale for the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    f"(got {args.gpus})"}))
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     from siconos_amd import numerics as N
     w = WORKLOADS[args.workload]
@@ -118,13 +149,17 @@ def main():
     ok_plain = sp.residual <= tol
     ok_fast = sf.residual <= tol
 
-    # sweep-kernel roofline: one instrumented plain solve, events around each
-    # colour launch on the library's stream
+    # sweep-kernel roofline: one instrumented plain solve; every colour launch
+    # carries a pair of HIP events stamped with the dispatch's own start and
+    # end on the library's stream
     H.set_reaction(np.zeros(3 * H.nc))
     si = H.solve(N.hip_params(tol=tol, max_iter=2000, instrument=1))
     achieved = si.sweep_kernel_bytes / (si.sweep_kernel_ms * 1e-3) / 1e9
+    pmc = PMC_TRAFFIC.get(args.workload)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                "frac": achieved / 8000.0, "traffic": None,
+                "frac": achieved / 8000.0,
+                "traffic": pmc["bytes_per_launch"] if pmc else None,
+                "traffic_source": pmc["source"] if pmc else None,
                 "kernel": "k_sweep<double> (one colour pass)",
                 "launch_avg_us": si.sweep_kernel_ms * 1e3 / max(si.sweep_kernel_launches, 1),
                 "bytes_per_launch_avg": si.sweep_kernel_bytes / max(si.sweep_kernel_launches, 1)}

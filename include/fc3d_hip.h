@@ -15,7 +15,16 @@
 
 #include <stdint.h>
 
+/* FC3D_HIP_STANDALONE: the device layer alone, for callers that bring their
+ * own numerics headers (the return codes are numerics_errors.h:41-67 plus
+ * NUMERICS_ERR_DEVICE = -60). */
+#ifdef FC3D_HIP_STANDALONE
+#ifndef FC3D_API
+#define FC3D_API __attribute__((visibility("default")))
+#endif
+#else
 #include "fc3d_numerics.h"
+#endif
 
 #ifdef __cplusplus
 extern "C" {

@@ -22,6 +22,8 @@
 #include <cstring>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "fc3d_internal.hpp"
 #include "fc3d_layout.hpp"
 #include "fc3d_sweep.hpp"
@@ -335,24 +337,40 @@ static const char* const kScheduleNames[FC3D_NUM_SCHEDULES + 1] = {
     "auto", "u4/u8 occ2", "u4/u8 occ3", "u4/u8 occ4", "u4/u8 pf16 occ2", "u4/u8 pf16 occ3",
     "u2/u4 pf8 occ4"};
 
+// With events, the launch goes through hipExtLaunchKernelGGL, which stamps
+// them with the dispatch's own start and end (the times a kernel-trace
+// profiler reports) instead of enqueuing separate marker packets.
 template <int U64, int U32, int PF, int MINW>
-static void launch_sweep_t(fc3d_hip_problem* h, int phase, int grid, int sb, int se) {
-  if (phase)
-    hipLaunchKernelGGL((k_sweep<float, U32, PF, MINW>), dim3(grid), dim3(256), 0, h->stream,
-                       ptrs32(h), h->prm, h->st, sb, se, 1);
-  else
-    hipLaunchKernelGGL((k_sweep<double, U64, PF, MINW>), dim3(grid), dim3(256), 0, h->stream,
-                       ptrs64(h, h->part), h->prm, h->st, sb, se, 0);
+static void launch_sweep_t(fc3d_hip_problem* h, int phase, int grid, int sb, int se,
+                           hipEvent_t e0, hipEvent_t e1) {
+  if (phase) {
+    auto k = k_sweep<float, U32, PF, MINW>;
+    if (e0)
+      hipExtLaunchKernelGGL(k, dim3(grid), dim3(256), 0, h->stream, e0, e1, 0, ptrs32(h), h->prm,
+                            (const SolveState*)h->st, sb, se, 1);
+    else
+      hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, h->stream, ptrs32(h), h->prm, h->st, sb, se,
+                         1);
+  } else {
+    auto k = k_sweep<double, U64, PF, MINW>;
+    if (e0)
+      hipExtLaunchKernelGGL(k, dim3(grid), dim3(256), 0, h->stream, e0, e1, 0,
+                            ptrs64(h, h->part), h->prm, (const SolveState*)h->st, sb, se, 0);
+    else
+      hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, h->stream, ptrs64(h, h->part), h->prm,
+                         h->st, sb, se, 0);
+  }
 }
 
-static void launch_sweep(fc3d_hip_problem* h, int phase, int schedule, int grid, int sb, int se) {
+static void launch_sweep(fc3d_hip_problem* h, int phase, int schedule, int grid, int sb, int se,
+                         hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   switch (schedule) {
-    case 2: launch_sweep_t<4, 8, 0, 3>(h, phase, grid, sb, se); break;
-    case 3: launch_sweep_t<4, 8, 0, 4>(h, phase, grid, sb, se); break;
-    case 4: launch_sweep_t<4, 8, 16, 2>(h, phase, grid, sb, se); break;
-    case 5: launch_sweep_t<4, 8, 16, 3>(h, phase, grid, sb, se); break;
-    case 6: launch_sweep_t<2, 4, 8, 4>(h, phase, grid, sb, se); break;
-    default: launch_sweep_t<4, 8, 0, 2>(h, phase, grid, sb, se); break;
+    case 2: launch_sweep_t<4, 8, 0, 3>(h, phase, grid, sb, se, e0, e1); break;
+    case 3: launch_sweep_t<4, 8, 0, 4>(h, phase, grid, sb, se, e0, e1); break;
+    case 4: launch_sweep_t<4, 8, 16, 2>(h, phase, grid, sb, se, e0, e1); break;
+    case 5: launch_sweep_t<4, 8, 16, 3>(h, phase, grid, sb, se, e0, e1); break;
+    case 6: launch_sweep_t<2, 4, 8, 4>(h, phase, grid, sb, se, e0, e1); break;
+    default: launch_sweep_t<4, 8, 0, 2>(h, phase, grid, sb, se, e0, e1); break;
   }
 }
 
@@ -361,9 +379,10 @@ static void enqueue_sweep(fc3d_hip_problem* h, int phase, int schedule, hipEvent
   for (int c = 0; c < h->ncolours; c++) {
     const int sb = h->colour_slice_ptr[(size_t)c], se = h->colour_slice_ptr[(size_t)c + 1];
     const int grid = (se - sb + 3) / 4;
-    if (ev && 2 * c + 1 < nev) hipEventRecord(ev[2 * c], h->stream);
-    launch_sweep(h, phase, schedule, grid, sb, se);
-    if (ev && 2 * c + 1 < nev) hipEventRecord(ev[2 * c + 1], h->stream);
+    if (ev && 2 * c + 1 < nev)
+      launch_sweep(h, phase, schedule, grid, sb, se, ev[2 * c], ev[2 * c + 1]);
+    else
+      launch_sweep(h, phase, schedule, grid, sb, se);
   }
   hipLaunchKernelGGL(k_finalize_sweep, dim3(1), dim3(1024), 0, h->stream, h->prm, h->st, h->part,
                      h->nslices, phase);
