@@ -54,7 +54,9 @@ typedef struct fc3d_hip_params {
   int hybrid_max_iter;       /* local iparam[16]                                  */
   /* additions */
   int precision;             /* 0: FP64 sweeps; 1: FP32 sweeps + FP64 refinement  */
-  int warm_start;            /* 0: resident reaction; 1: zero; 2: learned model   */
+  int warm_start;            /* 0: resident reaction; 1: zero; 2: learned model,
+                                kept only if its residual after one sweep is not
+                                above the zero start's                          */
   int schedule;              /* 0: autotuned; 1..6 explicit sweep-kernel schedule */
   int sweeps_per_launch;     /* sweeps captured per graph replay (>=1)            */
   double fp32_tol;           /* FP32 phase hands over when its light error < this */
@@ -75,6 +77,9 @@ typedef struct fc3d_hip_stats {
   int64_t sweep_kernel_launches;
   double sweep_kernel_bytes;  /* algorithmic bytes moved by those launches      */
   int schedule;               /* sweep-kernel schedule used (autotuned when 0 given) */
+  /* warm_start = 2: FP64 residual after one sweep from each start */
+  double start_residual_zero;
+  double start_residual_learned;
 } fc3d_hip_stats;
 
 FC3D_API int fc3d_hip_device_count(void);
@@ -157,6 +162,62 @@ FC3D_API int fc3d_hip_batch_get_velocity(const fc3d_hip_batch* b, double* veloci
 FC3D_API int fc3d_hip_batch_solve(fc3d_hip_batch* b, const fc3d_hip_params* p,
                                   fc3d_hip_batch_stats* s, int* info, int* iterations,
                                   double* residual);
+
+/* ---- learned warm start ----------------------------------------------------- */
+/* No reference counterpart.  A per-contact MLP (22 features -> 128 -> 128 ->
+ * 128 -> 3, GELU) predicts the converged reaction of each contact, scaled by
+ * |q_c| / W_cc(0,0), from its local problem.  Forward, backward and weight
+ * gradients run on the matrix cores (BF16 operands, FP32 accumulation); the
+ * FP32 master weights are updated by Adam on the GPU.
+ *
+ * Features of contact c (with s = |q_c|, w = W_cc(0,0), neighbours j != c of
+ * its block row, n the normal component):
+ *   0-2  q_c / s               3-11 W_cc / w (column-major)   12 mu_c
+ *   13   log w                 14   log s                     15 neighbour count
+ *   16   mean_j |q_j| / s      17   max_j |q_j| / s
+ *   18-20 (sum_j W_cj y_j) / s with y_j = (max(0, -q_jn) / W_jj(0,0), 0, 0)
+ *   21   fraction of neighbours with q_jn < 0
+ * Target: r_c * w / s.
+ *
+ * Parameters (FP32, FC3D_MLP_NPARAMS): W1[128][32] (inputs padded to 32, the
+ * last 10 columns stay zero), b1[128], W2[128][128], b2[128], W3[128][128],
+ * b3[128], W4[16][128] (rows 3..15 stay zero), b4[16]. */
+#define FC3D_MLP_FEATURES 22
+#define FC3D_MLP_HIDDEN 128
+#define FC3D_MLP_OUTPUTS 3
+#define FC3D_MLP_NPARAMS 39312
+
+typedef struct fc3d_hip_mlp fc3d_hip_mlp;
+
+/* Seeded He-uniform initialisation; feature mean 0 and std 1. */
+FC3D_API int fc3d_hip_mlp_create(uint64_t seed, int device, fc3d_hip_mlp** out);
+FC3D_API void fc3d_hip_mlp_destroy(fc3d_hip_mlp* m);
+FC3D_API int fc3d_hip_mlp_get_params(const fc3d_hip_mlp* m, float* params, float* mean,
+                                     float* stdv);
+/* Any pointer may be NULL (kept); stdv entries must be positive. */
+FC3D_API int fc3d_hip_mlp_set_params(fc3d_hip_mlp* m, const float* params, const float* mean,
+                                     const float* stdv);
+/* y[n][3] from raw features x[n][22] (standardised with the model's mean/std). */
+FC3D_API int fc3d_hip_mlp_forward(fc3d_hip_mlp* m, int n, const float* x, float* y);
+/* Mean squared error over n <= 65536 rows and its gradient w.r.t. every
+ * parameter (grads: FC3D_MLP_NPARAMS floats, may be NULL). */
+FC3D_API int fc3d_hip_mlp_gradients(fc3d_hip_mlp* m, int n, const float* x, const float* t,
+                                    float* grads, float* loss);
+/* Adam (lr, beta 0.9/0.999, eps 1e-8) over `epochs` seeded shuffles of the n
+ * samples in minibatches of `batch` rows (a positive multiple of 64).  With
+ * set_normalisation != 0 the feature mean/std are first taken from x.
+ * final_loss: mean squared error of the last minibatch. */
+FC3D_API int fc3d_hip_mlp_train(fc3d_hip_mlp* m, int n, const float* x, const float* t,
+                                int epochs, int batch, float lr, uint64_t seed,
+                                int set_normalisation, float* final_loss);
+/* Features of every contact (original order, [nc][22]) and, if target is not
+ * NULL, the scaled resident reaction ([nc][3]) as a training target. */
+FC3D_API int fc3d_hip_problem_features(fc3d_hip_problem* h, float* feat, float* target);
+/* The model used by warm_start = 2 on this problem (not owned; NULL detaches). */
+FC3D_API int fc3d_hip_problem_set_model(fc3d_hip_problem* h, fc3d_hip_mlp* m);
+/* The model fc3d_nsgs_hip attaches when iparam[WARM_START] = 2 (not owned). */
+FC3D_API void fc3d_hip_set_default_model(fc3d_hip_mlp* m);
+FC3D_API fc3d_hip_mlp* fc3d_hip_default_model(void);
 
 /* ---- colouring ----------------------------------------------------------- */
 /* Smallest colour not used by a lower-index neighbour, in index order;

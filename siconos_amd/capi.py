@@ -10,6 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libsiconos_amd.so")
 
 c_int_p = C.POINTER(C.c_int)
+c_float_p = C.POINTER(C.c_float)
 c_double_p = C.POINTER(C.c_double)
 c_int64_p = C.POINTER(C.c_int64)
 c_size_t_p = C.POINTER(C.c_size_t)
@@ -80,7 +81,8 @@ class HipStats(C.Structure):
                 ("residual", C.c_double), ("incr_error", C.c_double), ("num_colours", C.c_int),
                 ("warm_start_used", C.c_int), ("solve_ms", C.c_double),
                 ("sweep_kernel_ms", C.c_double), ("sweep_kernel_launches", C.c_int64),
-                ("sweep_kernel_bytes", C.c_double), ("schedule", C.c_int)]
+                ("sweep_kernel_bytes", C.c_double), ("schedule", C.c_int),
+                ("start_residual_zero", C.c_double), ("start_residual_learned", C.c_double)]
 
 
 class HipBatchStats(C.Structure):
@@ -173,6 +175,19 @@ SIGNATURES = {
     "fc3d_hip_local_solve": (C.c_int, [C.c_int, c_double_p, c_double_p, c_double_p, c_double_p,
                                        C.POINTER(HipParams), c_double_p, C.c_int, c_int_p]),
     "fc3d_hip_colour": (C.c_int, [C.c_int, c_int64_p, c_int64_p, c_int_p]),
+    "fc3d_hip_mlp_create": (C.c_int, [C.c_uint64, C.c_int, C.POINTER(H_p)]),
+    "fc3d_hip_mlp_destroy": (None, [H_p]),
+    "fc3d_hip_mlp_get_params": (C.c_int, [H_p, c_float_p, c_float_p, c_float_p]),
+    "fc3d_hip_mlp_set_params": (C.c_int, [H_p, c_float_p, c_float_p, c_float_p]),
+    "fc3d_hip_mlp_forward": (C.c_int, [H_p, C.c_int, c_float_p, c_float_p]),
+    "fc3d_hip_mlp_gradients": (C.c_int, [H_p, C.c_int, c_float_p, c_float_p, c_float_p,
+                                         c_float_p]),
+    "fc3d_hip_mlp_train": (C.c_int, [H_p, C.c_int, c_float_p, c_float_p, C.c_int, C.c_int,
+                                     C.c_float, C.c_uint64, C.c_int, c_float_p]),
+    "fc3d_hip_problem_features": (C.c_int, [H_p, c_float_p, c_float_p]),
+    "fc3d_hip_problem_set_model": (C.c_int, [H_p, H_p]),
+    "fc3d_hip_set_default_model": (None, [H_p]),
+    "fc3d_hip_default_model": (H_p, []),
     "fc3d_hip_batch_create": (C.c_int, [C.c_int, c_int_p, c_int64_p, c_int64_p, c_double_p,
                                         c_double_p, c_double_p, C.c_int, C.POINTER(H_p)]),
     "fc3d_hip_batch_destroy": (None, [H_p]),

@@ -26,6 +26,7 @@
 
 #include "fc3d_internal.hpp"
 #include "fc3d_layout.hpp"
+#include "fc3d_mlp.hpp"
 #include "fc3d_sweep.hpp"
 
 namespace fc3d {
@@ -213,6 +214,95 @@ __global__ __launch_bounds__(1024) void k_finalize_residual(const DevParams* prm
   if (st->notconv == 0 || st->iter >= prm->max_iter) st->stop = 1;
 }
 
+// Features of the learned warm start (defined with fc3d_hip_problem_features
+// in fc3d_hip.h) and, when tgt is not null, the resident reaction scaled by
+// W_cc(0,0) / |q_c| as a training target; one thread per slot.
+__device__ __forceinline__ void start_scales(const SweepPtrs<double>& P, int s, int lane, double* q,
+                                             double* W, double* sc, double* w0) {
+#pragma unroll
+  for (int e = 0; e < 3; e++) q[e] = P.q[(size_t)(s * 3 + e) * 64 + lane];
+#pragma unroll
+  for (int e = 0; e < 9; e++) W[e] = P.diag[(size_t)(s * 9 + e) * 64 + lane];
+  const double nq = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+  *sc = nq > 1e-300 ? nq : 1.0;
+  *w0 = W[0] > 1e-300 ? W[0] : 1.0;
+}
+
+__global__ void k_features(SweepPtrs<double> P, int nslots, float* __restrict__ feat,
+                           float* __restrict__ tgt) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= nslots) return;
+  const int s = slot >> 6, lane = slot & 63;
+  float* f = feat + (size_t)slot * MLP_NF;
+  const int len = P.len[slot];
+  if (len < 0) {
+    for (int k = 0; k < MLP_NF; k++) f[k] = 0.f;
+    if (tgt)
+      for (int e = 0; e < 3; e++) tgt[(size_t)slot * 3 + e] = 0.f;
+    return;
+  }
+  double q[3], W[9], sc, w0;
+  start_scales(P, s, lane, q, W, &sc, &w0);
+  double qsum = 0.0, qmax = 0.0, g[3] = {0.0, 0.0, 0.0};
+  int ncomp = 0;
+  const int koff = P.koff[s];
+  for (int k = 0; k < len; k++) {
+    const size_t plane = (size_t)(koff + k);
+    const int j = P.col[plane * 64 + lane];
+    const int sj = j >> 6, lj = j & 63;
+    const double qj0 = P.q[(size_t)(sj * 3) * 64 + lj];
+    const double qj1 = P.q[(size_t)(sj * 3 + 1) * 64 + lj];
+    const double qj2 = P.q[(size_t)(sj * 3 + 2) * 64 + lj];
+    const double nqj = sqrt(qj0 * qj0 + qj1 * qj1 + qj2 * qj2);
+    qsum += nqj;
+    qmax = fmax(qmax, nqj);
+    if (qj0 < 0.0) {
+      ncomp++;
+      const double wjj = P.diag[(size_t)(sj * 9) * 64 + lj];
+      const double yj = wjj > 1e-300 ? -qj0 / wjj : 0.0;
+      // first column of the column-major block W_cj times (yj, 0, 0)
+#pragma unroll
+      for (int e = 0; e < 3; e++) g[e] += P.vals[(plane * 9 + e) * 64 + lane] * yj;
+    }
+  }
+  for (int e = 0; e < 3; e++) f[e] = (float)(q[e] / sc);
+  for (int e = 0; e < 9; e++) f[3 + e] = (float)(W[e] / w0);
+  f[12] = (float)P.mu[slot];
+  f[13] = (float)log(w0);
+  f[14] = (float)log(sc);
+  f[15] = (float)len;
+  f[16] = (float)(len > 0 ? qsum / len / sc : 0.0);
+  f[17] = (float)(qmax / sc);
+  for (int e = 0; e < 3; e++) f[18 + e] = (float)(g[e] / sc);
+  f[21] = (float)(len > 0 ? (double)ncomp / len : 0.0);
+  if (tgt) {
+    const double4 r = P.r[slot];
+    tgt[(size_t)slot * 3] = (float)(r.x * w0 / sc);
+    tgt[(size_t)slot * 3 + 1] = (float)(r.y * w0 / sc);
+    tgt[(size_t)slot * 3 + 2] = (float)(r.z * w0 / sc);
+  }
+}
+
+// r = projection onto the friction cone of the network's scaled prediction
+__global__ void k_apply_start(SweepPtrs<double> P, int nslots, const float* __restrict__ y) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= nslots) return;
+  const int s = slot >> 6, lane = slot & 63;
+  if (P.len[slot] < 0) {
+    P.r[slot] = make_double4(0.0, 0.0, 0.0, 0.0);
+    return;
+  }
+  double q[3], W[9], sc, w0;
+  start_scales(P, s, lane, q, W, &sc, &w0);
+  double r[3];
+  for (int e = 0; e < 3; e++) {
+    const double v = (double)y[(size_t)slot * 3 + e] * sc / w0;
+    r[e] = isfinite(v) ? v : 0.0;
+  }
+  projection_on_cone<double>(r, P.mu[slot]);
+  P.r[slot] = make_double4(r[0], r[1], r[2], 0.0);
+}
+
 template <typename T>
 __global__ void k_fill(T* p, T v, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -276,6 +366,12 @@ struct fc3d_hip_problem {
   int graph_sweeps[2] = {0, 0};
   int graph_schedule[2] = {0, 0};
   int tuned_schedule[2] = {0, 0};  // autotuner's choice per phase (0: not tuned)
+  // learned warm start (model not owned)
+  fc3d_hip_mlp* model = nullptr;
+  float *feat = nullptr, *ypred = nullptr;
+  unsigned short* featX = nullptr;
+  double4* rbak = nullptr;
+  double* plibak = nullptr;
 };
 
 namespace fc3d {
@@ -417,6 +513,79 @@ static int final_residual(fc3d_hip_problem* h) {
                      ptrs64(h, h->part2), h->st, h->nslices, 1, nullptr, 0);
   hipLaunchKernelGGL(k_finalize_residual, dim3(1), dim3(1024), 0, h->stream, h->prm, h->st,
                      h->part2, h->nslices, 1);
+  FC3D_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// Learned warm start with its fallback: one sweep of `phase` from the zero
+// start and one from the network's start, each followed by the FP64
+// residual; the solve continues from the state with the smaller residual
+// (the zero start on a tie or a non-finite residual).  The kept sweep counts
+// as the solve's first; the other one is extra work inside the solve's time.
+static int learned_start(fc3d_hip_problem* h, int phase, int schedule, int max_iter,
+                         SolveState* s0, fc3d_hip_stats* out) {
+  const size_t ns = (size_t)h->nslots;
+  const unsigned nb = (unsigned)((ns + 255) / 256);
+  int rc;
+  if (!h->feat) {
+    if ((rc = dev_alloc(&h->feat, ns * MLP_NF)) || (rc = dev_alloc(&h->ypred, ns * 3)) ||
+        (rc = dev_alloc(&h->featX, (ns + 63) / 64 * 64 * MLP_K1)) ||
+        (rc = dev_alloc(&h->rbak, ns)) || (rc = dev_alloc(&h->plibak, ns)))
+      return rc;
+  }
+  hipLaunchKernelGGL(k_features, dim3(nb), dim3(256), 0, h->stream, ptrs64(h, h->part), h->nslots,
+                     h->feat, (float*)nullptr);
+  if ((rc = mlp_forward_device(h->model, h->nslots, h->feat, h->featX, h->ypred, h->stream)))
+    return rc;
+  double res[2];
+  for (int trial = 0; trial < 2; trial++) {
+    if (trial == 0)
+      hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)((4 * ns + 255) / 256)), dim3(256), 0,
+                         h->stream, reinterpret_cast<double*>(h->r), 0.0, 4 * ns);
+    else
+      hipLaunchKernelGGL(k_apply_start, dim3(nb), dim3(256), 0, h->stream, ptrs64(h, h->part),
+                         h->nslots, (const float*)h->ypred);
+    hipLaunchKernelGGL(k_fill<double>, dim3(nb), dim3(256), 0, h->stream, h->pli, 1.0, ns);
+    if (!s0->stop) {
+      if (phase)
+        hipLaunchKernelGGL(k_r64_to_r32, dim3(nb), dim3(256), 0, h->stream, h->r, h->r32, h->pli,
+                           h->pli32, ns);
+      for (int c = 0; c < h->ncolours; c++) {
+        const int sb = h->colour_slice_ptr[(size_t)c], se = h->colour_slice_ptr[(size_t)c + 1];
+        launch_sweep(h, phase, schedule, (se - sb + 3) / 4, sb, se);
+      }
+      if (phase)
+        hipLaunchKernelGGL(k_r32_to_r64, dim3(nb), dim3(256), 0, h->stream, h->r32, h->r,
+                           h->pli32, h->pli, ns);
+    }
+    if ((rc = final_residual(h))) return rc;
+    SolveState cur;
+    FC3D_HIP_TRY(hipMemcpyAsync(&cur, h->st, sizeof cur, hipMemcpyDeviceToHost, h->stream));
+    FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
+    res[trial] = cur.full;
+    if (trial == 0) {
+      FC3D_HIP_TRY(hipMemcpyAsync(h->rbak, h->r, sizeof(double4) * ns, hipMemcpyDeviceToDevice,
+                                  h->stream));
+      FC3D_HIP_TRY(hipMemcpyAsync(h->plibak, h->pli, sizeof(double) * ns,
+                                  hipMemcpyDeviceToDevice, h->stream));
+    }
+  }
+  const int keep = res[1] < res[0];
+  if (!keep) {
+    FC3D_HIP_TRY(hipMemcpyAsync(h->r, h->rbak, sizeof(double4) * ns, hipMemcpyDeviceToDevice,
+                                h->stream));
+    FC3D_HIP_TRY(hipMemcpyAsync(h->pli, h->plibak, sizeof(double) * ns, hipMemcpyDeviceToDevice,
+                                h->stream));
+  }
+  out->warm_start_used = keep ? 2 : 1;
+  out->start_residual_zero = res[0];
+  out->start_residual_learned = res[1];
+  if (!s0->stop) {
+    s0->iter = 1;
+    s0->fp32_iter = phase ? 1 : 0;
+    s0->stop = s0->iter >= max_iter ? 1 : 0;
+  }
+  FC3D_HIP_TRY(hipMemcpyAsync(h->st, s0, sizeof *s0, hipMemcpyHostToDevice, h->stream));
   FC3D_HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -615,7 +784,7 @@ void fc3d_hip_problem_destroy(fc3d_hip_problem* h) {
   void* ptrs[] = {h->len,  h->dpos,   h->koff,   h->kmax, h->col,  h->vals,  h->diag,
                   h->q,    h->mu,     h->pli,    h->r,    h->u,    h->x,     h->vals32,
                   h->diag32, h->q32,  h->mu32,   h->pli32, h->r32, h->part,  h->part2,
-                  h->prm,  h->st};
+                  h->prm,  h->st,    h->feat,   h->ypred, h->featX, h->rbak, h->plibak};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (h->stream) hipStreamDestroy(h->stream);
@@ -709,6 +878,50 @@ int fc3d_hip_gemv(fc3d_hip_problem* h, const double* x, double* y) {
   return get_vec(h, h->u, y);
 }
 
+int fc3d_hip_problem_features(fc3d_hip_problem* h, float* feat, float* target) {
+  if (!h || !feat) return NUMERICS_ERR_NULL_POINTER;
+  int rc = use_device(h->device);
+  if (rc) return rc;
+  const size_t ns = (size_t)h->nslots;
+  float *df = nullptr, *dt = nullptr;
+  if ((rc = dev_alloc(&df, ns * MLP_NF)) || (rc = dev_alloc(&dt, ns * 3))) {
+    hipFree(df);
+    return rc;
+  }
+  hipLaunchKernelGGL(k_features, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, h->stream,
+                     ptrs64(h, h->part), h->nslots, df, target ? dt : (float*)nullptr);
+  std::vector<float> hf(ns * MLP_NF), ht(target ? ns * 3 : 0);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(hf.data(), df, sizeof(float) * hf.size(), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess && target)
+    e = hipMemcpyAsync(ht.data(), dt, sizeof(float) * ht.size(), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  hipFree(df);
+  hipFree(dt);
+  if (e != hipSuccess) {
+    set_error("fc3d_hip_problem_features: %s", hipGetErrorString(e));
+    return NUMERICS_ERR_DEVICE;
+  }
+  for (int i = 0; i < h->nc; i++) {
+    const size_t slot = (size_t)h->orig2slot[(size_t)i];
+    std::memcpy(feat + (size_t)i * MLP_NF, hf.data() + slot * MLP_NF, sizeof(float) * MLP_NF);
+    if (target) std::memcpy(target + (size_t)i * 3, ht.data() + slot * 3, sizeof(float) * 3);
+  }
+  return 0;
+}
+
+int fc3d_hip_problem_set_model(fc3d_hip_problem* h, fc3d_hip_mlp* m) {
+  if (!h) return NUMERICS_ERR_NULL_POINTER;
+  if (m && mlp_device(m) != h->device) {
+    set_error("fc3d_hip_problem_set_model: model on device %d, problem on device %d",
+              mlp_device(m), h->device);
+    return NUMERICS_ERR_INVALID_ARGUMENT;
+  }
+  h->model = m;
+  return 0;
+}
+
 int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats* s) {
   if (!h || !p || !s) return NUMERICS_ERR_NULL_POINTER;
   const int ls = p->local_solver;
@@ -725,9 +938,13 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
     set_error("fc3d_hip_solve: rho strategy %d not supported", p->rho_strategy);
     return NUMERICS_ERR_NOT_IMPLEMENTED;
   }
-  if (p->warm_start == 2) {
-    set_error("fc3d_hip_solve: learned warm start needs a trained model (fc3d_hip_mlp_*)");
+  if (p->warm_start < 0 || p->warm_start > 2) {
+    set_error("fc3d_hip_solve: warm_start %d is not 0, 1 or 2", p->warm_start);
     return NUMERICS_ERR_INVALID_ARGUMENT;
+  }
+  if (p->warm_start == 2 && !h->model) {
+    set_error("fc3d_hip_solve: warm_start 2 needs a model (fc3d_hip_problem_set_model)");
+    return NUMERICS_ERR_INVALID_OPTION;
   }
   int rc = use_device(h->device);
   if (rc) return rc;
@@ -788,6 +1005,8 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
     for (auto& e : kev) FC3D_HIP_TRY(hipEventCreate(&e));
   }
   FC3D_HIP_TRY(hipEventRecord(t0, h->stream));
+  if (p->warm_start == 2 && (rc = learned_start(h, mixed, sched[mixed], p->max_iter, &s0, s)))
+    return rc;
   const int chunk = p->sweeps_per_launch > 0 ? p->sweeps_per_launch : 1;
   // FC3D_HIP_NO_GRAPH=1 enqueues the same kernels directly instead of
   // replaying a captured graph (for tools that do not follow graph launches)

@@ -40,6 +40,10 @@ def dptr(a):
     return a.ctypes.data_as(capi.c_double_p)
 
 
+def fptr(a):
+    return a.ctypes.data_as(capi.c_float_p)
+
+
 def iptr(a):
     return a.ctypes.data_as(capi.c_int_p)
 
@@ -347,10 +351,93 @@ class HipProblem:
         self._check(self._lib.fc3d_hip_gemv(self._h, dptr(x), dptr(y)), "gemv")
         return y
 
+    def features(self, target=True):
+        """(features [nc, 22], scaled resident reaction [nc, 3] or None)."""
+        f = np.empty((self.nc, MLP_FEATURES), dtype=np.float32)
+        t = np.empty((self.nc, 3), dtype=np.float32) if target else None
+        self._check(self._lib.fc3d_hip_problem_features(
+            self._h, fptr(f), fptr(t) if target else None), "features")
+        return f, t
+
+    def set_model(self, model):
+        """Model for warm_start=2 (kept alive by this object)."""
+        self._model = model
+        self._check(self._lib.fc3d_hip_problem_set_model(
+            self._h, model._h if model is not None else None), "set_model")
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
             self._lib.fc3d_hip_problem_destroy(h)
+            self._h = None
+
+
+MLP_FEATURES = 22
+MLP_NPARAMS = 39312
+
+
+class Mlp:
+    """The warm-start network of fc3d_hip.h (fc3d_hip_mlp_*)."""
+
+    def __init__(self, seed=0, device=0):
+        self._lib = capi.load()
+        h = C.c_void_p()
+        rc = self._lib.fc3d_hip_mlp_create(seed, device, C.byref(h))
+        if rc:
+            raise RuntimeError(f"fc3d_hip_mlp_create: {rc} {capi.last_error()}")
+        self._h = h
+
+    def _check(self, rc, what):
+        if rc:
+            raise RuntimeError(f"{what}: {rc} {capi.last_error()}")
+
+    def params(self):
+        p = np.empty(MLP_NPARAMS, dtype=np.float32)
+        m = np.empty(MLP_FEATURES, dtype=np.float32)
+        sd = np.empty(MLP_FEATURES, dtype=np.float32)
+        self._check(self._lib.fc3d_hip_mlp_get_params(self._h, fptr(p), fptr(m), fptr(sd)),
+                    "mlp_get_params")
+        return p, m, sd
+
+    def set_params(self, params=None, mean=None, std=None):
+        a = [None if v is None else np.ascontiguousarray(v, dtype=np.float32)
+             for v in (params, mean, std)]
+        self._check(self._lib.fc3d_hip_mlp_set_params(
+            self._h, *[None if v is None else fptr(v) for v in a]), "mlp_set_params")
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.empty((len(x), 3), dtype=np.float32)
+        self._check(self._lib.fc3d_hip_mlp_forward(self._h, len(x), fptr(x), fptr(y)),
+                    "mlp_forward")
+        return y
+
+    def gradients(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.empty(MLP_NPARAMS, dtype=np.float32)
+        loss = C.c_float()
+        self._check(self._lib.fc3d_hip_mlp_gradients(self._h, len(x), fptr(x), fptr(t), fptr(g),
+                                                     C.byref(loss)), "mlp_gradients")
+        return g, loss.value
+
+    def train(self, x, t, epochs=30, batch=1024, lr=1e-3, seed=0, set_normalisation=True):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = C.c_float()
+        self._check(self._lib.fc3d_hip_mlp_train(self._h, len(x), fptr(x), fptr(t), epochs,
+                                                 batch, lr, seed, int(set_normalisation),
+                                                 C.byref(loss)), "mlp_train")
+        return loss.value
+
+    def set_default(self):
+        """Model used by fc3d_driver / fc3d_nsgs_hip when iparam[21] = 2."""
+        self._lib.fc3d_hip_set_default_model(self._h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.fc3d_hip_mlp_destroy(h)
             self._h = None
 
 
