@@ -4,8 +4,11 @@ Workload (default): the 1M-contact granular pile (400k spheres, mu ~ U[0.1,0.9],
 v_free ~ N(0,1) scaled by 0.1 on 10% of the bodies), solved to an FP64
 fc3d_compute_error <= 1e-6.  One step = one complete solve from a zero
 reaction with the data resident in HBM.  The reported method ("fast") is the
-FP32-sweep + FP64-refinement coloured NSGS; the plain FP64 coloured NSGS is
-timed beside it on the same input.
+coloured NSGS with the learned warm start, FP32 sweeps accelerated by
+Anderson mixing on alternate sweeps, and FP64 refinement; the plain FP64
+coloured NSGS (reference options, zero start) is
+timed beside it on the same input, and so is the fast method without the
+learned start (ablation).
 
 Prints one JSON line (rank 0).  `--gpus N` with N > 1 is refused: this build
 runs on one GPU.
@@ -89,6 +92,22 @@ def cpu_baseline(g, sweeps_needed, sample_sweeps):
                        f"{os.cpu_count()} logical cpus")}
 
 
+def train_warm_start(w, tol):
+    """The warm-start network trained on 200k converged contacts of four
+    problems of the workload's generator with other seeds."""
+    from siconos_amd import numerics as N
+    from siconos_amd import warmstart
+    if w["gen"] == "pile":
+        probs = (N.generate_granular_pile(20_000, 50_000, w["mu"][0], w["mu"][1], 1000 + k)
+                 for k in range(4))
+    else:
+        probs = (N.generate_random_pairs(20_000, 50_000, w["mu"][0], w["mu"][1], 1000 + k)
+                 for k in range(4))
+    x, t, _ = warmstart.collect(((g.W, g.q, g.mu) for g in probs), tol=tol)
+    model, loss = warmstart.train(x, t, seed=1, epochs=30, batch=1024, lr=1e-3)
+    return model, loss, len(x)
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of known size: its per-dispatch
     FETCH_SIZE and WRITE_SIZE readings convert the counters to bytes."""
@@ -135,17 +154,32 @@ def main():
     setup_s = time.perf_counter() - t0
     log(f"[bench] device problem ready in {setup_s:.1f} s; colours={H.num_colours}")
 
-    plain = N.hip_params(tol=tol, max_iter=2000, sweeps_per_launch=16)
-    fast = N.hip_params(tol=tol, max_iter=2000, sweeps_per_launch=16, precision=1,
-                        fp32_tol=1e-6)
+    # the learned warm start: trained (outside the timed region) on converged
+    # contacts of other problems of the same generator
+    t0 = time.perf_counter()
+    model, mlp_loss, train_nc = train_warm_start(w, tol)
+    H.set_model(model)
+    log(f"[bench] warm-start network trained on {train_nc} contacts in "
+        f"{time.perf_counter() - t0:.1f} s (loss {mlp_loss:.4f})")
+
+    plain = N.hip_params(tol=tol, max_iter=3000, sweeps_per_launch=16)
+    fast_kw = dict(tol=tol, max_iter=3000, sweeps_per_launch=16, precision=1, fp32_tol=1e-6,
+                   accel_depth=3, accel_every=2)
+    fast = N.hip_params(warm_start=2, **fast_kw)
+    fast_nomlp = N.hip_params(warm_start=1, **fast_kw)
     log("[bench] timing plain FP64 coloured NSGS")
     tp, devp, sp = time_method(H, plain, args.steps, args.warmup)
     log(f"[bench] plain: {tp / args.steps * 1e3:.1f} ms/solve, sweeps={sp.iterations}, "
         f"residual={sp.residual:.3e}")
-    log("[bench] timing FP32 sweeps + FP64 refinement")
+    log("[bench] timing learned start + FP32 Anderson sweeps + FP64 refinement")
     tf, devf, sf = time_method(H, fast, args.steps, args.warmup)
     log(f"[bench] fast: {tf / args.steps * 1e3:.1f} ms/solve, sweeps={sf.iterations} "
-        f"(fp32 {sf.fp32_sweeps}), residual={sf.residual:.3e}")
+        f"(fp32 {sf.fp32_sweeps}, anderson {sf.accel_steps}, restarts {sf.accel_restarts}), "
+        f"start used {sf.warm_start_used}, residual={sf.residual:.3e}")
+    log("[bench] timing the same without the learned start")
+    tn, devn, sn = time_method(H, fast_nomlp, args.steps, args.warmup)
+    log(f"[bench] fast without network: {tn / args.steps * 1e3:.1f} ms/solve, "
+        f"sweeps={sn.iterations}, residual={sn.residual:.3e}")
     ok_plain = sp.residual <= tol
     ok_fast = sf.residual <= tol
 
@@ -180,11 +214,20 @@ def main():
         "dtype": "f32 sweeps + f64 residual/refinement", "data": "synthetic",
         "config": {"workload": w["desc"], "contacts": nc, "bodies": w["nbodies"],
                    "nnz_blocks": int(g.W.nnzb), "colours": H.num_colours, "tol": tol,
-                   "method": "coloured NSGS, FP32 sweeps + FP64 refinement"},
+                   "method": "coloured NSGS: learned warm start (fallback to zero), FP32 "
+                             "sweeps with Anderson(3) on alternate sweeps, FP64 refinement"},
         "plain_fp64": {"value": plain_value, "ms_per_step": tp / args.steps * 1e3,
                        "sweeps": sp.iterations, "residual": sp.residual},
         "fast": {"sweeps": sf.iterations, "fp32_sweeps": sf.fp32_sweeps,
+                 "anderson_steps": sf.accel_steps, "anderson_restarts": sf.accel_restarts,
+                 "warm_start_used": {1: "zero (fallback)", 2: "learned"}.get(sf.warm_start_used),
+                 "residual_after_first_sweep": {"zero": sf.start_residual_zero,
+                                                "learned": sf.start_residual_learned},
                  "residual": sf.residual},
+        "fast_without_network": {"value": nc * args.steps / tn if sn.residual <= tol else 0.0,
+                                 "ms_per_step": tn / args.steps * 1e3, "sweeps": sn.iterations,
+                                 "residual": sn.residual},
+        "warm_start_training": {"contacts": train_nc, "final_loss": mlp_loss},
         "speedup_vs_plain": (value / plain_value) if plain_value else None,
         "roofline": roofline, "cpu_baseline": cpu,
     }

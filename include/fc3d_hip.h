@@ -62,6 +62,12 @@ typedef struct fc3d_hip_params {
   double fp32_tol;           /* FP32 phase hands over when its light error < this */
   double fp32_local_tol;     /* local tolerance used by FP32 sweeps               */
   int instrument;            /* 1: time each sweep-kernel launch with HIP events  */
+  /* Anderson acceleration of the sweep map (no reference counterpart):
+   * x <- G(x) - dG gamma with the last accel_depth differences, on every
+   * accel_every-th sweep, history restarted when |G(x) - x| grows. */
+  int accel_depth;           /* 0: off (default); 1..8 history columns           */
+  int accel_every;           /* Anderson step on every n-th sweep (default 2)     */
+  double accel_reg;          /* Tikhonov weight relative to the Gram trace (1e-10)*/
 } fc3d_hip_params;
 
 typedef struct fc3d_hip_stats {
@@ -80,6 +86,8 @@ typedef struct fc3d_hip_stats {
   /* warm_start = 2: FP64 residual after one sweep from each start */
   double start_residual_zero;
   double start_residual_learned;
+  int accel_steps;            /* Anderson steps taken                            */
+  int accel_restarts;         /* history restarts                                */
 } fc3d_hip_stats;
 
 FC3D_API int fc3d_hip_device_count(void);

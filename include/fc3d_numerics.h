@@ -251,11 +251,14 @@ enum SICONOS_FRICTION_3D_NSGS_HIP_IPARAM {
   SICONOS_FRICTION_3D_NSGS_HIP_DEVICE = 24,      /* HIP device ordinal                  */
   SICONOS_FRICTION_3D_NSGS_HIP_NUM_COLOURS = 25, /* out: colours of the contact graph   */
   SICONOS_FRICTION_3D_NSGS_HIP_FP32_SWEEPS = 26, /* out: sweeps done in FP32             */
-  SICONOS_FRICTION_3D_NSGS_HIP_WARM_START_USED = 27 /* out: start actually used         */
+  SICONOS_FRICTION_3D_NSGS_HIP_WARM_START_USED = 27, /* out: start actually used        */
+  SICONOS_FRICTION_3D_NSGS_HIP_ACCEL_DEPTH = 28,  /* Anderson history columns, 0 = off  */
+  SICONOS_FRICTION_3D_NSGS_HIP_ACCEL_EVERY = 29   /* Anderson step every n-th sweep      */
 };
 enum SICONOS_FRICTION_3D_NSGS_HIP_DPARAM {
   SICONOS_FRICTION_3D_NSGS_HIP_FP32_TOL = 23,    /* residual at which FP32 hands over   */
-  SICONOS_FRICTION_3D_NSGS_HIP_TIME_SOLVE = 24   /* out: device time of sweeps, ms      */
+  SICONOS_FRICTION_3D_NSGS_HIP_TIME_SOLVE = 24,  /* out: device time of sweeps, ms      */
+  SICONOS_FRICTION_3D_NSGS_HIP_ACCEL_REG = 25    /* Anderson regularisation (x trace)    */
 };
 
 /* ------------------------------------------------------------------------ */

@@ -73,7 +73,8 @@ class HipParams(C.Structure):
                 ("hybrid_strategy", C.c_int), ("hybrid_max_loop", C.c_int),
                 ("hybrid_max_iter", C.c_int), ("precision", C.c_int), ("warm_start", C.c_int),
                 ("schedule", C.c_int), ("sweeps_per_launch", C.c_int), ("fp32_tol", C.c_double),
-                ("fp32_local_tol", C.c_double), ("instrument", C.c_int)]
+                ("fp32_local_tol", C.c_double), ("instrument", C.c_int),
+                ("accel_depth", C.c_int), ("accel_every", C.c_int), ("accel_reg", C.c_double)]
 
 
 class HipStats(C.Structure):
@@ -82,7 +83,8 @@ class HipStats(C.Structure):
                 ("warm_start_used", C.c_int), ("solve_ms", C.c_double),
                 ("sweep_kernel_ms", C.c_double), ("sweep_kernel_launches", C.c_int64),
                 ("sweep_kernel_bytes", C.c_double), ("schedule", C.c_int),
-                ("start_residual_zero", C.c_double), ("start_residual_learned", C.c_double)]
+                ("start_residual_zero", C.c_double), ("start_residual_learned", C.c_double),
+                ("accel_steps", C.c_int), ("accel_restarts", C.c_int)]
 
 
 class HipBatchStats(C.Structure):

@@ -509,7 +509,12 @@ int fc3d_hip_batch_solve(fc3d_hip_batch* b, const fc3d_hip_params* p, fc3d_hip_b
   int rc = use_device(b->device);
   if (rc) return rc;
   std::memset(s, 0, sizeof *s);
+  if (p->accel_depth != 0) {
+    set_error("fc3d_hip_batch_solve: Anderson acceleration is not available for batches");
+    return NUMERICS_ERR_NOT_IMPLEMENTED;
+  }
   DevParams d;
+  std::memset(&d, 0, sizeof d);
   d.lp.solver = p->local_solver;
   d.lp.newton_max_iter = p->local_max_iter;
   d.lp.formulation = p->local_formulation;

@@ -25,6 +25,7 @@
 #include <hip/hip_ext.h>
 
 #include "fc3d_internal.hpp"
+#include "fc3d_accel.hpp"
 #include "fc3d_layout.hpp"
 #include "fc3d_mlp.hpp"
 #include "fc3d_sweep.hpp"
@@ -134,6 +135,7 @@ __global__ __launch_bounds__(1024) void k_finalize_sweep(const DevParams* prm, S
   const double nr2 = block_sum(part, nslices, 2, 1, sh);
   if (threadIdx.x != 0) return;
   st->iter += 1;
+  st->nf = sqrt(light);
   double incr = sqrt(light);
   const double nr = sqrt(nr2);
   if (fabs(nr) > DBL_EPSILON) incr /= nr;
@@ -372,6 +374,13 @@ struct fc3d_hip_problem {
   unsigned short* featX = nullptr;
   double4* rbak = nullptr;
   double* plibak = nullptr;
+  // Anderson acceleration (allocated on first use)
+  int accel_on = 0;
+  int aa_nblocks = 0;
+  AccelBufs<double> aa64 = {};
+  AccelBufs<float> aa32 = {};
+  AccelState* ast = nullptr;
+  int graph_accel[2] = {0, 0};
 };
 
 namespace fc3d {
@@ -441,20 +450,22 @@ static void launch_sweep_t(fc3d_hip_problem* h, int phase, int grid, int sb, int
                            hipEvent_t e0, hipEvent_t e1) {
   if (phase) {
     auto k = k_sweep<float, U32, PF, MINW>;
+    SweepPtrs<float> P = ptrs32(h);
+    if (h->accel_on) P.f = h->aa32.f;
     if (e0)
-      hipExtLaunchKernelGGL(k, dim3(grid), dim3(256), 0, h->stream, e0, e1, 0, ptrs32(h), h->prm,
+      hipExtLaunchKernelGGL(k, dim3(grid), dim3(256), 0, h->stream, e0, e1, 0, P, h->prm,
                             (const SolveState*)h->st, sb, se, 1);
     else
-      hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, h->stream, ptrs32(h), h->prm, h->st, sb, se,
-                         1);
+      hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, h->stream, P, h->prm, h->st, sb, se, 1);
   } else {
     auto k = k_sweep<double, U64, PF, MINW>;
+    SweepPtrs<double> P = ptrs64(h, h->part);
+    if (h->accel_on) P.f = h->aa64.f;
     if (e0)
-      hipExtLaunchKernelGGL(k, dim3(grid), dim3(256), 0, h->stream, e0, e1, 0,
-                            ptrs64(h, h->part), h->prm, (const SolveState*)h->st, sb, se, 0);
+      hipExtLaunchKernelGGL(k, dim3(grid), dim3(256), 0, h->stream, e0, e1, 0, P, h->prm,
+                            (const SolveState*)h->st, sb, se, 0);
     else
-      hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, h->stream, ptrs64(h, h->part), h->prm,
-                         h->st, sb, se, 0);
+      hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, h->stream, P, h->prm, h->st, sb, se, 0);
   }
 }
 
@@ -488,10 +499,80 @@ static void enqueue_sweep(fc3d_hip_problem* h, int phase, int schedule, hipEvent
     hipLaunchKernelGGL(k_finalize_residual, dim3(1), dim3(1024), 0, h->stream, h->prm, h->st,
                        h->part2, h->nslices, 0);
   }
+  if (h->accel_on) {
+    const dim3 g((unsigned)h->aa_nblocks), b(256);
+    if (phase) {
+      hipLaunchKernelGGL(k_aa_update<float>, g, b, 0, h->stream, h->r32, h->aa32, h->st, h->ast,
+                         h->prm, h->nslots, 1);
+      hipLaunchKernelGGL(k_aa_finalize, dim3(1), dim3(1024), 0, h->stream, h->aa32.part,
+                         h->aa_nblocks, h->st, h->ast, h->prm, 1);
+      hipLaunchKernelGGL(k_aa_apply<float>, g, b, 0, h->stream, h->r32, h->aa32, h->st, h->ast,
+                         h->prm, h->nslots, 1);
+    } else {
+      hipLaunchKernelGGL(k_aa_update<double>, g, b, 0, h->stream, h->r, h->aa64, h->st, h->ast,
+                         h->prm, h->nslots, 0);
+      hipLaunchKernelGGL(k_aa_finalize, dim3(1), dim3(1024), 0, h->stream, h->aa64.part,
+                         h->aa_nblocks, h->st, h->ast, h->prm, 0);
+      hipLaunchKernelGGL(k_aa_apply<double>, g, b, 0, h->stream, h->r, h->aa64, h->st, h->ast,
+                         h->prm, h->nslots, 0);
+    }
+  }
+}
+
+// Anderson buffers of one phase, allocated once (f zeroed: the sweep never
+// writes padding slots).
+template <typename T>
+static int alloc_accel(fc3d_hip_problem* h, AccelBufs<T>* A) {
+  if (A->f) return 0;
+  const size_t ns = (size_t)h->nslots;
+  using V = typename vec4<T>::type;
+  int rc;
+  if ((rc = dev_alloc(&A->f, ns)) || (rc = dev_alloc(&A->fprev, ns)) ||
+      (rc = dev_alloc(&A->gprev, ns)) || (rc = dev_alloc(&A->dF, ns * AA_MAX)) ||
+      (rc = dev_alloc(&A->dG, ns * AA_MAX)) ||
+      (rc = dev_alloc(&A->part, (size_t)h->aa_nblocks * 2 * AA_MAX)))
+    return rc;
+  FC3D_HIP_TRY(hipMemset(A->f, 0, sizeof(V) * ns));
+  return 0;
+}
+
+static int ensure_accel(fc3d_hip_problem* h, int mixed) {
+  h->aa_nblocks = (h->nslots + 255) / 256;
+  int rc;
+  if (!h->ast && (rc = dev_alloc(&h->ast, 1))) return rc;
+  if ((rc = alloc_accel(h, &h->aa64))) return rc;
+  if (mixed && (rc = alloc_accel(h, &h->aa32))) return rc;
+  return 0;
+}
+
+static void free_accel(fc3d_hip_problem* h) {
+  void* ptrs[] = {h->aa64.f,  h->aa64.fprev, h->aa64.gprev, h->aa64.dF, h->aa64.dG,
+                  h->aa64.part, h->aa32.f,   h->aa32.fprev, h->aa32.gprev, h->aa32.dF,
+                  h->aa32.dG, h->aa32.part, h->ast};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+}
+
+static int accel_stats(fc3d_hip_problem* h, fc3d_hip_stats* s) {
+  AccelState a;
+  FC3D_HIP_TRY(hipMemcpyAsync(&a, h->ast, sizeof a, hipMemcpyDeviceToHost, h->stream));
+  FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
+  s->accel_steps += a.steps;
+  s->accel_restarts += a.resets;
+  return 0;
+}
+
+static int reset_accel(fc3d_hip_problem* h) {
+  AccelState a;
+  std::memset(&a, 0, sizeof a);
+  a.nf_prev = INFINITY;
+  FC3D_HIP_TRY(hipMemcpyAsync(h->ast, &a, sizeof a, hipMemcpyHostToDevice, h->stream));
+  return 0;
 }
 
 static int build_graph(fc3d_hip_problem* h, int phase, int sweeps, int schedule) {
-  if (h->graph[phase] && h->graph_sweeps[phase] == sweeps && h->graph_schedule[phase] == schedule)
+  if (h->graph[phase] && h->graph_sweeps[phase] == sweeps && h->graph_schedule[phase] == schedule &&
+      h->graph_accel[phase] == h->accel_on)
     return 0;
   if (h->graph[phase]) {
     hipGraphExecDestroy(h->graph[phase]);
@@ -505,6 +586,7 @@ static int build_graph(fc3d_hip_problem* h, int phase, int sweeps, int schedule)
   hipGraphDestroy(g);
   h->graph_sweeps[phase] = sweeps;
   h->graph_schedule[phase] = schedule;
+  h->graph_accel[phase] = h->accel_on;
   return 0;
 }
 
@@ -655,6 +737,10 @@ static int tune_schedule(fc3d_hip_problem* h, int phase, double norm_q, double l
 
 static DevParams make_params(const fc3d_hip_params* p) {
   DevParams d;
+  std::memset(&d, 0, sizeof d);
+  d.accel_depth = p->accel_depth;
+  d.accel_every = p->accel_every;
+  d.accel_reg = p->accel_reg;
   d.lp.solver = p->local_solver;
   d.lp.newton_max_iter = p->local_max_iter;
   d.lp.formulation = p->local_formulation;
@@ -787,6 +873,7 @@ void fc3d_hip_problem_destroy(fc3d_hip_problem* h) {
                   h->prm,  h->st,    h->feat,   h->ypred, h->featX, h->rbak, h->plibak};
   for (void* p : ptrs)
     if (p) hipFree(p);
+  free_accel(h);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
 }
@@ -942,6 +1029,12 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
     set_error("fc3d_hip_solve: warm_start %d is not 0, 1 or 2", p->warm_start);
     return NUMERICS_ERR_INVALID_ARGUMENT;
   }
+  if (p->accel_depth < 0 || p->accel_depth > AA_MAX || p->accel_every < 1 ||
+      !(p->accel_reg >= 0.0)) {
+    set_error("fc3d_hip_solve: acceleration depth %d (0..%d), every %d (>= 1), reg %g (>= 0)",
+              p->accel_depth, AA_MAX, p->accel_every, p->accel_reg);
+    return NUMERICS_ERR_INVALID_ARGUMENT;
+  }
   if (p->warm_start == 2 && !h->model) {
     set_error("fc3d_hip_solve: warm_start 2 needs a model (fc3d_hip_problem_set_model)");
     return NUMERICS_ERR_INVALID_OPTION;
@@ -962,6 +1055,7 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
   const int mixed = p->precision == 1;
   if (mixed && (rc = ensure_fp32(h))) return rc;
 
+  h->accel_on = 0;  // the autotuner times plain sweeps
   int sched[2] = {p->schedule, p->schedule};
   if (p->schedule < 0 || p->schedule > FC3D_NUM_SCHEDULES) {
     set_error("fc3d_hip_solve: schedule %d out of range [0, %d]", p->schedule, FC3D_NUM_SCHEDULES);
@@ -982,6 +1076,10 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
     }
   }
   s->schedule = sched[mixed];
+  if (p->accel_depth > 0) {
+    if ((rc = ensure_accel(h, mixed))) return rc;
+    h->accel_on = 1;
+  }
 
   SolveState s0;
   std::memset(&s0, 0, sizeof s0);
@@ -1028,6 +1126,10 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
     } else if (phase == 1) {
       hipLaunchKernelGGL(k_r64_to_r32, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0,
                          h->stream, h->r, h->r32, h->pli, h->pli32, ns);
+    }
+    if (h->accel_on) {
+      if (phase == 0 && mixed && (rc = accel_stats(h, s))) return rc;
+      if ((rc = reset_accel(h))) return rc;
     }
     // every replay advances the sweep counter or stops; the bound only
     // guards against a launch that silently did not run
@@ -1084,6 +1186,7 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
     full = fin.full;
     notconv = full < p->tol ? 0 : 1;
   }
+  if (h->accel_on && (rc = accel_stats(h, s))) return rc;
   float ms = 0.f;
   hipEventElapsedTime(&ms, t0, t1);
   hipEventDestroy(t0);

@@ -64,6 +64,9 @@ void fc3d_hip_params_default(fc3d_hip_params* p) {
   p->fp32_tol = 1e-5;
   p->fp32_local_tol = 1e-6;
   p->instrument = 0;
+  p->accel_depth = 0;
+  p->accel_every = 2;
+  p->accel_reg = 1e-10;
 }
 
 }  // extern "C"

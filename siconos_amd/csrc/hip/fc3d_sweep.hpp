@@ -18,6 +18,9 @@ struct DevParams {
   double fp32_tol;
   double omega;
   double rho_constant;
+  double accel_reg;     // Anderson: Tikhonov weight relative to the Gram trace
+  int accel_depth;      // Anderson history columns (0: off)
+  int accel_every;      // Anderson step on every accel_every-th sweep
   int mode;             // error evaluation (NonSmoothGaussSeidel_options.h:121-140)
   int freq;             // error evaluation frequency
   int max_iter;
@@ -40,6 +43,7 @@ struct SolveState {
   double working_tol;
   double local_tol;
   double norm_q;
+  double nf;      // sqrt of the last sweep's sum |r_new - r_old|^2
 };
 
 template <typename T>
@@ -57,6 +61,7 @@ struct SweepPtrs {
   typename vec4<T>::type* u;
   T* pli;           // per-slot PLI step (persistent local state)
   double* part;     // per-slice partial sums
+  typename vec4<T>::type* f = nullptr;  // if set: r_new - r_old of the sweep
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -185,6 +190,11 @@ __device__ __forceinline__ void sweep_slot(const SweepPtrs<T>& P, const DevParam
     if (accept) { out.x = lr[0]; out.y = lr[1]; out.z = lr[2]; }
     P.r[slot] = out;
     P.pli[slot] = pli;
+    if (P.f) {
+      typename vec4<T>::type d = out;
+      d.x -= ro.x; d.y -= ro.y; d.z -= ro.z; d.w = T(0);
+      P.f[slot] = d;
+    }
     *nr2 = sizeof(T) == 8 ? double(out.x * out.x + out.y * out.y + out.z * out.z)
                           : double(out.x) * double(out.x) + double(out.y) * double(out.y) +
                                 double(out.z) * double(out.z);

@@ -57,6 +57,11 @@ int fc3d_nsgs_hip_params_from_options(const SolverOptions* o, fc3d_hip_params* p
     p->sweeps_per_launch = o->iparam[SICONOS_FRICTION_3D_NSGS_HIP_SWEEPS_PER_LAUNCH];
   if (o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_FP32_TOL] > 0.0)
     p->fp32_tol = o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_FP32_TOL];
+  p->accel_depth = o->iparam[SICONOS_FRICTION_3D_NSGS_HIP_ACCEL_DEPTH];
+  if (o->iparam[SICONOS_FRICTION_3D_NSGS_HIP_ACCEL_EVERY] > 0)
+    p->accel_every = o->iparam[SICONOS_FRICTION_3D_NSGS_HIP_ACCEL_EVERY];
+  if (o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_ACCEL_REG] > 0.0)
+    p->accel_reg = o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_ACCEL_REG];
   return 0;
 }
 
@@ -129,6 +134,9 @@ void fc3d_nsgs_hip_set_default(SolverOptions* o) {
   o->iparam[SICONOS_FRICTION_3D_NSGS_HIP_SWEEPS_PER_LAUNCH] = 8;
   o->iparam[SICONOS_FRICTION_3D_NSGS_HIP_DEVICE] = 0;
   o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_FP32_TOL] = 1e-5;
+  o->iparam[SICONOS_FRICTION_3D_NSGS_HIP_ACCEL_DEPTH] = 0;
+  o->iparam[SICONOS_FRICTION_3D_NSGS_HIP_ACCEL_EVERY] = 2;
+  o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_ACCEL_REG] = 1e-10;
 }
 
 static int nsgs_hip_solve_wrap(void* problem, double* reaction, double* velocity,
