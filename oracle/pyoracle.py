@@ -178,5 +178,36 @@ def greedy_colouring(W):
     return c, n
 
 
+def solve_many(problems, tol, max_iter, nthreads):
+    """oracle_solve_many: every (W, q, mu) from a zero start in natural order,
+    one problem per thread; returns (converged, iterations, residuals, r)."""
+    lib = load()
+    nc = np.array([W.nc for W, _, _ in problems], dtype=np.int32)
+    rp_off = np.zeros(len(problems), dtype=np.int64)
+    b_off = np.zeros(len(problems), dtype=np.int64)
+    rps, cis, bls = [], [], []
+    ro = bo = 0
+    for k, (W, _, _) in enumerate(problems):
+        rp_off[k] = ro
+        b_off[k] = bo
+        rps.append(np.asarray(W.row_ptr, dtype=np.int64) + bo)
+        cis.append(np.asarray(W.col_idx, dtype=np.int64))
+        bls.append(np.asarray(W.values, dtype=np.float64).ravel())
+        ro += W.nc + 1
+        bo += W.nnzb
+    rp = np.ascontiguousarray(np.concatenate(rps))
+    ci = np.ascontiguousarray(np.concatenate(cis))
+    bl = np.ascontiguousarray(np.concatenate(bls))
+    q = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64) for _, x, _ in problems]))
+    mu = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64) for _, _, x in problems]))
+    r = np.zeros(len(q))
+    iters = np.zeros(len(problems), dtype=np.int32)
+    res = np.zeros(len(problems))
+    ok = lib.oracle_solve_many(len(problems), i32(nc), i64(rp_off), i64(b_off), i64(rp), i64(ci),
+                               d(bl), d(q), d(mu), d(r), tol, max_iter, nthreads, i32(iters),
+                               d(res))
+    return ok, iters, res, r
+
+
 def mutation(m):
     load().oracle_set_mutation(m)

@@ -136,3 +136,17 @@ def test_trivial_case(oracle):
     assert info == 0 and it == 0
     np.testing.assert_array_equal(r, 0.0)
     np.testing.assert_array_equal(u, q)
+
+
+def test_solve_many_equals_one_solve_per_problem(oracle):
+    """The batched CPU baseline (oracle_solve_many, OpenMP over problems)
+    returns what one fc3d_driver call per problem returns."""
+    probs = N.generate_batch(12, 8, 60, 0.1, 0.9, 4)
+    ok, iters, res, r = oracle.solve_many(probs, 1e-6, 1000, 4)
+    off = 0
+    for k, (W, q, mu) in enumerate(probs):
+        info, rk, uk, it, rs = oracle.nsgs(W, q, mu, tol=1e-6, max_iter=1000)
+        assert it == iters[k] and rs == res[k]
+        np.testing.assert_array_equal(r[3 * off:3 * (off + W.nc)], rk)
+        off += W.nc
+    assert ok == sum(1 for x in res if x <= 1e-6)
