@@ -140,4 +140,53 @@ void append_problem(HostLayout& L, int c0, int n, const int64_t* row_ptr, const 
   }
 }
 
+void build_dataflow(const HostLayout& L, const int* colour, DataflowHost* out) {
+  const int ns = L.nslices;
+  std::vector<std::vector<int>> deps((size_t)ns);
+  for (int s = 0; s < ns; s++) {
+    const int kmax = L.kmax[(size_t)s];
+    for (int lane = 0; lane < 64; lane++) {
+      const int slot = s * 64 + lane;
+      const int i = L.slot2orig[(size_t)slot];
+      if (i < 0) continue;
+      const int len = L.len[(size_t)slot];
+      for (int k = 0; k < len && k < kmax; k++) {
+        const int js = L.col[((size_t)L.koff[(size_t)s] + (size_t)k) * 64 + (size_t)lane];
+        const int j = L.slot2orig[(size_t)js];
+        const int sj = js / 64;
+        if (j < 0 || sj == s) continue;
+        if (colour[j] < colour[i])
+          deps[(size_t)s].push_back(sj);
+        else if (colour[j] > colour[i])
+          deps[(size_t)sj].push_back(s);
+      }
+    }
+  }
+  out->dep_ptr.assign((size_t)ns + 1, 0);
+  out->dep_idx.clear();
+  for (int s = 0; s < ns; s++) {
+    auto& d = deps[(size_t)s];
+    std::sort(d.begin(), d.end());
+    d.erase(std::unique(d.begin(), d.end()), d.end());
+    out->dep_idx.insert(out->dep_idx.end(), d.begin(), d.end());
+    out->dep_ptr[(size_t)s + 1] = (int)out->dep_idx.size();
+  }
+  // per-XCD queues, colour-major inside each queue
+  std::vector<std::vector<int>> q(8);
+  const int ncol = (int)L.colour_slice_ptr.size() - 1;
+  for (int c = 0; c < ncol; c++) {
+    const int sb = L.colour_slice_ptr[(size_t)c], se = L.colour_slice_ptr[(size_t)c + 1];
+    const int n = se - sb;
+    for (int x = 0; x < 8; x++)
+      for (int t = sb + (int)((int64_t)n * x / 8); t < sb + (int)((int64_t)n * (x + 1) / 8); t++)
+        q[(size_t)x].push_back(t);
+  }
+  out->qlist.clear();
+  for (int x = 0; x < 8; x++) {
+    out->qoff[x] = (int)out->qlist.size();
+    out->qlist.insert(out->qlist.end(), q[(size_t)x].begin(), q[(size_t)x].end());
+  }
+  out->qoff[8] = (int)out->qlist.size();
+}
+
 }  // namespace fc3d

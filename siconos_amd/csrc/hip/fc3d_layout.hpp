@@ -36,4 +36,18 @@ void append_problem(HostLayout& L, int c0, int n, const int64_t* row_ptr, const 
                     const double* blocks, const double* q, const double* mu, const int* colour,
                     const RowInfo& info, int window, int* ncolours_out);
 
+// Dependencies of the single-launch (dataflow) sweep.  A slice may start once
+// every slice holding a neighbour of lower colour has finished the current
+// sweep; neighbours of higher colour wait for it in turn (edges are taken
+// from both directions of the sparsity, so a non-symmetric pattern is safe).
+// Slices are dealt from one queue per XCD; each queue lists, colour after
+// colour, a contiguous eighth of the colour's slices.
+struct DataflowHost {
+  std::vector<int> dep_ptr;  // nslices + 1
+  std::vector<int> dep_idx;
+  std::vector<int> qlist;    // nslices: the queues one after the other
+  int qoff[9];               // queue q = qlist[qoff[q] .. qoff[q + 1])
+};
+void build_dataflow(const HostLayout& L, const int* colour, DataflowHost* out);
+
 }  // namespace fc3d

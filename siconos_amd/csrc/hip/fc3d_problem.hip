@@ -57,6 +57,89 @@ __global__ __launch_bounds__(256, MINW) void k_sweep(SweepPtrs<T> P,
   }
 }
 
+// Single-launch sweep: every colour in one launch, a slice starting as soon
+// as the slices holding its lower-colour neighbours have finished this sweep
+// (their reactions are then final for it, and its higher-colour neighbours
+// wait for it in turn, so every contact sees exactly the values of the
+// colour-by-colour sweep).  Waves take slices from their XCD's queue in
+// colour-major order; a wave only ever waits on slices dealt before its own,
+// each held by a running wave, so the lowest unfinished slice always
+// proceeds.  Completion is published with an agent-scope release (L2
+// write-back) and observed with an acquire (L2 invalidate) before the
+// neighbour reads.  A wait longer than DF_SPIN_LIMIT polls sets an error
+// flag and proceeds, so the launch always drains.
+struct DataflowState {
+  int epoch;      // mark of the current sweep; only ever grows (starts at 1)
+  int error;      // a dependency wait exceeded its bound
+  int ticket[8];  // per-queue position of the current sweep
+};
+
+struct DFArgs {
+  const int* dep_ptr;
+  const int* dep_idx;
+  const int* qlist;
+  int* done;  // per slice: mark of the last sweep it completed
+  DataflowState* df;
+  int qoff[9];
+};
+
+constexpr int DF_SPIN_LIMIT = 1 << 20;
+
+template <typename T, int U, int PF, int MINW>
+__global__ __launch_bounds__(256, MINW) void k_sweep_df(SweepPtrs<T> P,
+                                                        const DevParams* __restrict__ prm,
+                                                        const SolveState* __restrict__ st,
+                                                        DFArgs D, int want_phase) {
+  if (st->stop || st->phase != want_phase) return;
+  const int lane = threadIdx.x & 63;
+  const int x = blockIdx.x & 7;
+  const int qb = D.qoff[x], qn = D.qoff[x + 1] - qb;
+  const int epoch = D.df->epoch;
+  const T tol = T(want_phase ? prm->fp32_local_tol : st->local_tol);
+  for (;;) {
+    int t = 0;
+    if (lane == 0) t = atomicAdd(&D.df->ticket[x], 1);
+    t = __shfl(t, 0, 64);
+    if (t >= qn) break;
+    const int s = __builtin_amdgcn_readfirstlane(D.qlist[qb + t]);
+    const int d0 = D.dep_ptr[s], d1 = D.dep_ptr[s + 1];
+    for (int b = d0; b < d1; b += 64) {
+      const int i = b + lane;
+      const int dep = i < d1 ? D.dep_idx[i] : -1;
+      for (int spins = 0;; spins++) {
+        const bool ok = dep < 0 || __hip_atomic_load(&D.done[dep], __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT) >= epoch;
+        if (__builtin_amdgcn_ballot_w64(!ok) == 0) break;
+        if (spins >= DF_SPIN_LIMIT) {
+          if (lane == 0) atomicOr(&D.df->error, 1);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    double light, nr2;
+    sweep_slot<T, U, PF>(P, prm, s, lane, tol, &light, &nr2);
+    light = wave_sum(light);
+    nr2 = wave_sum(nr2);
+    if (lane == 0) {
+      P.part[2 * (size_t)s] = light;
+      P.part[2 * (size_t)s + 1] = nr2;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane == 0)
+      __hip_atomic_store(&D.done[s], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// After each single-launch sweep: next mark, queues rewound.
+__global__ void k_df_advance(DataflowState* df) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    df->epoch += 1;
+    for (int x = 0; x < 8; x++) df->ticket[x] = 0;
+  }
+}
+
 // Residual pass (fc3d_compute_error.c:70-104) over all slices, or, with
 // gemv_only, y += W x on the resident layout (SparseBlockMatrix.c:360-419).
 __global__ __launch_bounds__(256) void k_residual(SweepPtrs<double> P, const SolveState* st,
@@ -381,6 +464,12 @@ struct fc3d_hip_problem {
   AccelBufs<float> aa32 = {};
   AccelState* ast = nullptr;
   int graph_accel[2] = {0, 0};
+  // single-launch sweep (schedule 7), set up on first use
+  DataflowHost dfh;
+  int df_ready = 0;
+  int df_grid = 0;
+  int *dep_ptr = nullptr, *dep_idx = nullptr, *qlist = nullptr, *done = nullptr;
+  DataflowState* df = nullptr;
 };
 
 namespace fc3d {
@@ -437,10 +526,11 @@ static int ensure_fp32(fc3d_hip_problem* h) {
 
 // Enqueues one sweep of `phase` (1 FP32, 0 FP64) plus its convergence step.
 // Sweep-kernel schedules: unroll U, column prefetch PF, occupancy target.
-#define FC3D_NUM_SCHEDULES 6
+#define FC3D_NUM_SCHEDULES 7
+#define FC3D_SCHEDULE_DATAFLOW 7
 static const char* const kScheduleNames[FC3D_NUM_SCHEDULES + 1] = {
     "auto", "u4/u8 occ2", "u4/u8 occ3", "u4/u8 occ4", "u4/u8 pf16 occ2", "u4/u8 pf16 occ3",
-    "u2/u4 pf8 occ4"};
+    "u2/u4 pf8 occ4", "single launch, u4/u8 occ2"};
 
 // With events, the launch goes through hipExtLaunchKernelGGL, which stamps
 // them with the dispatch's own start and end (the times a kernel-trace
@@ -477,19 +567,71 @@ static void launch_sweep(fc3d_hip_problem* h, int phase, int schedule, int grid,
     case 4: launch_sweep_t<4, 8, 16, 2>(h, phase, grid, sb, se, e0, e1); break;
     case 5: launch_sweep_t<4, 8, 16, 3>(h, phase, grid, sb, se, e0, e1); break;
     case 6: launch_sweep_t<2, 4, 8, 4>(h, phase, grid, sb, se, e0, e1); break;
-    default: launch_sweep_t<4, 8, 0, 2>(h, phase, grid, sb, se, e0, e1); break;
+    default: launch_sweep_t<4, 8, 0, 2>(h, phase, grid, sb, se, e0, e1); break;  // 1 and 7
   }
+}
+
+static DFArgs df_args(fc3d_hip_problem* h) {
+  DFArgs D;
+  D.dep_ptr = h->dep_ptr;
+  D.dep_idx = h->dep_idx;
+  D.qlist = h->qlist;
+  D.done = h->done;
+  D.df = h->df;
+  for (int x = 0; x < 9; x++) D.qoff[x] = h->dfh.qoff[x];
+  return D;
+}
+
+static void launch_sweep_df(fc3d_hip_problem* h, int phase, hipEvent_t e0, hipEvent_t e1) {
+  const dim3 g((unsigned)h->df_grid), b(256);
+  const DFArgs D = df_args(h);
+  if (phase) {
+    auto k = k_sweep_df<float, 8, 0, 2>;
+    SweepPtrs<float> P = ptrs32(h);
+    if (h->accel_on) P.f = h->aa32.f;
+    if (e0)
+      hipExtLaunchKernelGGL(k, g, b, 0, h->stream, e0, e1, 0, P, h->prm, (const SolveState*)h->st,
+                            D, 1);
+    else
+      hipLaunchKernelGGL(k, g, b, 0, h->stream, P, h->prm, h->st, D, 1);
+  } else {
+    auto k = k_sweep_df<double, 4, 0, 2>;
+    SweepPtrs<double> P = ptrs64(h, h->part);
+    if (h->accel_on) P.f = h->aa64.f;
+    if (e0)
+      hipExtLaunchKernelGGL(k, g, b, 0, h->stream, e0, e1, 0, P, h->prm, (const SolveState*)h->st,
+                            D, 0);
+    else
+      hipLaunchKernelGGL(k, g, b, 0, h->stream, P, h->prm, h->st, D, 0);
+  }
+  hipLaunchKernelGGL(k_df_advance, dim3(1), dim3(64), 0, h->stream, h->df);
+}
+
+// Event pairs one timed sweep uses, and the algorithmic bytes of pair c.
+static int timed_launches(const fc3d_hip_problem* h, int schedule) {
+  return schedule == FC3D_SCHEDULE_DATAFLOW ? 1 : h->ncolours;
+}
+static double timed_bytes(const fc3d_hip_problem* h, int phase, int schedule, int c) {
+  const std::vector<double>& b = phase ? h->colour_bytes32 : h->colour_bytes64;
+  if (schedule != FC3D_SCHEDULE_DATAFLOW) return b[(size_t)c];
+  double t = 0.0;
+  for (double v : b) t += v;
+  return t;
 }
 
 static void enqueue_sweep(fc3d_hip_problem* h, int phase, int schedule, hipEvent_t* ev,
                           int nev) {
-  for (int c = 0; c < h->ncolours; c++) {
-    const int sb = h->colour_slice_ptr[(size_t)c], se = h->colour_slice_ptr[(size_t)c + 1];
-    const int grid = (se - sb + 3) / 4;
-    if (ev && 2 * c + 1 < nev)
-      launch_sweep(h, phase, schedule, grid, sb, se, ev[2 * c], ev[2 * c + 1]);
-    else
-      launch_sweep(h, phase, schedule, grid, sb, se);
+  if (schedule == FC3D_SCHEDULE_DATAFLOW) {
+    launch_sweep_df(h, phase, ev && nev >= 2 ? ev[0] : nullptr, ev && nev >= 2 ? ev[1] : nullptr);
+  } else {
+    for (int c = 0; c < h->ncolours; c++) {
+      const int sb = h->colour_slice_ptr[(size_t)c], se = h->colour_slice_ptr[(size_t)c + 1];
+      const int grid = (se - sb + 3) / 4;
+      if (ev && 2 * c + 1 < nev)
+        launch_sweep(h, phase, schedule, grid, sb, se, ev[2 * c], ev[2 * c + 1]);
+      else
+        launch_sweep(h, phase, schedule, grid, sb, se);
+    }
   }
   hipLaunchKernelGGL(k_finalize_sweep, dim3(1), dim3(1024), 0, h->stream, h->prm, h->st, h->part,
                      h->nslices, phase);
@@ -559,6 +701,39 @@ static int accel_stats(fc3d_hip_problem* h, fc3d_hip_stats* s) {
   FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
   s->accel_steps += a.steps;
   s->accel_restarts += a.resets;
+  return 0;
+}
+
+// Uploads the dependency lists and queues of the single-launch sweep.
+static int ensure_dataflow(fc3d_hip_problem* h) {
+  if (h->df_ready) return 0;
+  int rc;
+  int cus = 0;
+  FC3D_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+  h->df_grid = std::max(8, (cus * 2) / 8 * 8);  // two 4-wave blocks per CU, a multiple of 8
+  if ((rc = dev_upload(&h->dep_ptr, h->dfh.dep_ptr)) ||
+      (rc = dev_upload(&h->dep_idx, h->dfh.dep_idx)) ||
+      (rc = dev_upload(&h->qlist, h->dfh.qlist)) || (rc = dev_alloc(&h->done, (size_t)h->nslices)) ||
+      (rc = dev_alloc(&h->df, 1)))
+    return rc;
+  FC3D_HIP_TRY(hipMemset(h->done, 0, sizeof(int) * (size_t)h->nslices));
+  DataflowState d;
+  std::memset(&d, 0, sizeof d);
+  d.epoch = 1;
+  FC3D_HIP_TRY(hipMemcpy(h->df, &d, sizeof d, hipMemcpyHostToDevice));
+  h->df_ready = 1;
+  return 0;
+}
+
+static int dataflow_error(fc3d_hip_problem* h) {
+  if (!h->df_ready) return 0;
+  DataflowState d;
+  FC3D_HIP_TRY(hipMemcpyAsync(&d, h->df, sizeof d, hipMemcpyDeviceToHost, h->stream));
+  FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
+  if (d.error) {
+    set_error("single-launch sweep: a dependency wait exceeded its bound");
+    return NUMERICS_ERR_DEVICE;
+  }
   return 0;
 }
 
@@ -709,6 +884,10 @@ static int tune_schedule(fc3d_hip_problem* h, int phase, double norm_q, double l
   double best_ms = 1e300;
   *best = 1;
   for (int sc = 1; sc <= FC3D_NUM_SCHEDULES; sc++) {
+    if (sc == FC3D_SCHEDULE_DATAFLOW) {
+      int rc = ensure_dataflow(h);
+      if (rc) return rc;
+    }
     FC3D_HIP_TRY(hipMemcpyAsync(h->st, &t, sizeof t, hipMemcpyHostToDevice, h->stream));
     FC3D_HIP_TRY(hipMemcpyAsync(live, saved, bytes, hipMemcpyDeviceToDevice, h->stream));
     double ms_total = 0.0;
@@ -717,7 +896,7 @@ static int tune_schedule(fc3d_hip_problem* h, int phase, double norm_q, double l
       FC3D_HIP_TRY(hipGetLastError());
       FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
       if (rep == 0) continue;  // first sweep warms caches and code
-      for (int c = 0; c < h->ncolours; c++) {
+      for (int c = 0; c < timed_launches(h, sc); c++) {
         float ms = 0.f;
         hipEventElapsedTime(&ms, ev[2 * (size_t)c], ev[2 * (size_t)c + 1]);
         ms_total += ms;
@@ -731,6 +910,10 @@ static int tune_schedule(fc3d_hip_problem* h, int phase, double norm_q, double l
   for (auto& e : ev) hipEventDestroy(e);
   FC3D_HIP_TRY(hipMemcpyAsync(live, saved, bytes, hipMemcpyDeviceToDevice, h->stream));
   FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
+  {
+    int rc = dataflow_error(h);
+    if (rc) return rc;
+  }
   g_tune_cache.push_back({{lg, h->ncolours, phase}, *best});
   return 0;
 }
@@ -841,6 +1024,7 @@ int fc3d_hip_problem_create(int nc, const int64_t* row_ptr, const int64_t* col_i
   h->orig2slot = L.orig2slot;
   h->colour_bytes64 = L.colour_bytes64;
   h->colour_bytes32 = L.colour_bytes32;
+  build_dataflow(L, h->colour.data(), &h->dfh);
   if ((rc = dev_upload(&h->len, L.len)) || (rc = dev_upload(&h->dpos, L.dpos)) ||
       (rc = dev_upload(&h->koff, L.koff)) || (rc = dev_upload(&h->kmax, L.kmax)) ||
       (rc = dev_upload(&h->col, L.col)) || (rc = dev_upload(&h->vals, L.vals)) ||
@@ -874,6 +1058,9 @@ void fc3d_hip_problem_destroy(fc3d_hip_problem* h) {
   for (void* p : ptrs)
     if (p) hipFree(p);
   free_accel(h);
+  void* dfp[] = {h->dep_ptr, h->dep_idx, h->qlist, h->done, h->df};
+  for (void* p : dfp)
+    if (p) hipFree(p);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
 }
@@ -1076,6 +1263,9 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
     }
   }
   s->schedule = sched[mixed];
+  if ((sched[0] == FC3D_SCHEDULE_DATAFLOW || (mixed && sched[1] == FC3D_SCHEDULE_DATAFLOW)) &&
+      (rc = ensure_dataflow(h)))
+    return rc;
   if (p->accel_depth > 0) {
     if ((rc = ensure_accel(h, mixed))) return rc;
     h->accel_on = 1;
@@ -1145,15 +1335,12 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
         FC3D_HIP_TRY(hipGetLastError());
         FC3D_HIP_TRY(hipMemcpyAsync(&cur, h->st, sizeof cur, hipMemcpyDeviceToHost, h->stream));
         FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
-        {
-          for (int c = 0; c < h->ncolours; c++) {
-            float ms = 0.f;
-            hipEventElapsedTime(&ms, kev[2 * (size_t)c], kev[2 * (size_t)c + 1]);
-            s->sweep_kernel_ms += ms;
-            s->sweep_kernel_launches += 1;
-            s->sweep_kernel_bytes +=
-                phase ? h->colour_bytes32[(size_t)c] : h->colour_bytes64[(size_t)c];
-          }
+        for (int c = 0; c < timed_launches(h, sched[phase]); c++) {
+          float ms = 0.f;
+          hipEventElapsedTime(&ms, kev[2 * (size_t)c], kev[2 * (size_t)c + 1]);
+          s->sweep_kernel_ms += ms;
+          s->sweep_kernel_launches += 1;
+          s->sweep_kernel_bytes += timed_bytes(h, phase, sched[phase], c);
         }
       } else if (no_graph) {
         for (int k = 0; k < chunk; k++) enqueue_sweep(h, phase, sched[phase], nullptr, 0);
@@ -1187,6 +1374,7 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
     notconv = full < p->tol ? 0 : 1;
   }
   if (h->accel_on && (rc = accel_stats(h, s))) return rc;
+  if ((rc = dataflow_error(h))) return rc;
   float ms = 0.f;
   hipEventElapsedTime(&ms, t0, t1);
   hipEventDestroy(t0);

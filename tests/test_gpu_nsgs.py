@@ -192,3 +192,36 @@ def test_problem_create_rejects_bad_structure():
     ci[5] = 10 ** 6
     with pytest.raises(RuntimeError, match="out of range"):
         N.HipProblem(N.SparseBlock3(W.row_ptr, ci, W.values), g.q, g.mu)
+
+
+@pytest.fixture(scope="module")
+def pile50k_reference(oracle):
+    g = N.generate_granular_pile(20000, 50000, 0.1, 0.9, 5)
+    H = N.HipProblem(g.W, g.q, g.mu)
+    info, ro, uo, it, res = oracle.nsgs(g.W, g.q, g.mu, tol=1e-6, max_iter=1000, order=H.order())
+    assert info == 0
+    return g, H, ro, it
+
+
+@pytest.mark.parametrize("schedule", range(1, 8))
+def test_every_sweep_schedule_is_bit_exact(pile50k_reference, schedule):
+    """Schedules 1-6 launch one kernel per colour with different unrolling,
+    prefetch and occupancy; schedule 7 runs all colours in one launch, each
+    slice waiting for its lower-colour neighbours.  All of them must give the
+    oracle's colour-order result bit for bit."""
+    g, H, ro, it = pile50k_reference
+    H.set_reaction(np.zeros(3 * g.nc))
+    s = H.solve(N.hip_params(tol=1e-6, max_iter=1000, schedule=schedule))
+    assert s.schedule == schedule and s.info == 0 and s.iterations == it
+    np.testing.assert_array_equal(H.reaction(), ro)
+
+
+def test_single_launch_sweep_with_fp32_anderson_and_timing(pile50k_reference):
+    g, H, ro, it = pile50k_reference
+    s = H.solve(N.hip_params(tol=1e-6, max_iter=3000, schedule=7, precision=1, fp32_tol=1e-6,
+                             accel_depth=3, warm_start=1))
+    assert s.info == 0 and s.residual <= 1e-6
+    s = H.solve(N.hip_params(tol=1e-6, max_iter=1000, schedule=7, warm_start=1, instrument=1))
+    assert s.info == 0 and s.iterations == it
+    assert s.sweep_kernel_launches == it  # one launch per sweep
+    np.testing.assert_array_equal(H.reaction(), ro)
