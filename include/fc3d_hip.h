@@ -57,7 +57,7 @@ typedef struct fc3d_hip_params {
   int warm_start;            /* 0: resident reaction; 1: zero; 2: learned model,
                                 kept only if its residual after one sweep is not
                                 above the zero start's                          */
-  int schedule;              /* 0: autotuned; 1..6 explicit sweep-kernel schedule */
+  int schedule;              /* 0: autotuned; 1..9 explicit (7: single launch)    */
   int sweeps_per_launch;     /* sweeps captured per graph replay (>=1)            */
   double fp32_tol;           /* FP32 phase hands over when its light error < this */
   double fp32_local_tol;     /* local tolerance used by FP32 sweeps               */

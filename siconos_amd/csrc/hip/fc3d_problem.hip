@@ -526,11 +526,11 @@ static int ensure_fp32(fc3d_hip_problem* h) {
 
 // Enqueues one sweep of `phase` (1 FP32, 0 FP64) plus its convergence step.
 // Sweep-kernel schedules: unroll U, column prefetch PF, occupancy target.
-#define FC3D_NUM_SCHEDULES 7
+#define FC3D_NUM_SCHEDULES 9
 #define FC3D_SCHEDULE_DATAFLOW 7
 static const char* const kScheduleNames[FC3D_NUM_SCHEDULES + 1] = {
     "auto", "u4/u8 occ2", "u4/u8 occ3", "u4/u8 occ4", "u4/u8 pf16 occ2", "u4/u8 pf16 occ3",
-    "u2/u4 pf8 occ4", "single launch, u4/u8 occ2"};
+    "u2/u4 pf8 occ4", "single launch, u4/u8 occ2", "u8/u16 occ1", "u8/u16 pf16 occ1"};
 
 // With events, the launch goes through hipExtLaunchKernelGGL, which stamps
 // them with the dispatch's own start and end (the times a kernel-trace
@@ -567,6 +567,8 @@ static void launch_sweep(fc3d_hip_problem* h, int phase, int schedule, int grid,
     case 4: launch_sweep_t<4, 8, 16, 2>(h, phase, grid, sb, se, e0, e1); break;
     case 5: launch_sweep_t<4, 8, 16, 3>(h, phase, grid, sb, se, e0, e1); break;
     case 6: launch_sweep_t<2, 4, 8, 4>(h, phase, grid, sb, se, e0, e1); break;
+    case 8: launch_sweep_t<8, 16, 0, 1>(h, phase, grid, sb, se, e0, e1); break;
+    case 9: launch_sweep_t<8, 16, 16, 1>(h, phase, grid, sb, se, e0, e1); break;
     default: launch_sweep_t<4, 8, 0, 2>(h, phase, grid, sb, se, e0, e1); break;  // 1 and 7
   }
 }
