@@ -272,7 +272,7 @@ def main():
             "value": nc * args.steps / tn if sn.residual <= tol else 0.0,
             "ms_per_step": tn / args.steps * 1e3, "sweeps": sn.iterations,
             "warm_start_used": {1: "zero (fallback)", 2: "learned"}.get(sn.warm_start_used),
-            "residual_after_first_sweep": {"zero": sn.start_residual_zero,
+            "residual_after_first_sweep": {"zero": sn.start_residual_fallback,
                                            "learned": sn.start_residual_learned},
             "residual": sn.residual},
         "warm_start_training": {"contacts": train_nc, "final_loss": mlp_loss},

@@ -55,8 +55,9 @@ typedef struct fc3d_hip_params {
   /* additions */
   int precision;             /* 0: FP64 sweeps; 1: FP32 sweeps + FP64 refinement  */
   int warm_start;            /* 0: resident reaction; 1: zero; 2: learned model,
-                                kept only if its residual after one sweep is not
-                                above the zero start's                          */
+                                kept only if its FP64 residual after one sweep is
+                                below the zero start's; 3: the same against the
+                                resident reaction (previous time step)          */
   int schedule;              /* 0: autotuned; 1..9 explicit (7: single launch)    */
   int sweeps_per_launch;     /* sweeps captured per graph replay (>=1)            */
   double fp32_tol;           /* FP32 phase hands over when its light error < this */
@@ -83,8 +84,9 @@ typedef struct fc3d_hip_stats {
   int64_t sweep_kernel_launches;
   double sweep_kernel_bytes;  /* algorithmic bytes moved by those launches      */
   int schedule;               /* sweep-kernel schedule used (autotuned when 0 given) */
-  /* warm_start = 2: FP64 residual after one sweep from each start */
-  double start_residual_zero;
+  /* warm_start = 2, 3: FP64 residual after one sweep from the fallback start
+   * (zero, or the resident reaction) and from the learned start */
+  double start_residual_fallback;
   double start_residual_learned;
   int accel_steps;            /* Anderson steps taken                            */
   int accel_restarts;         /* history restarts                                */
@@ -221,7 +223,7 @@ FC3D_API int fc3d_hip_mlp_train(fc3d_hip_mlp* m, int n, const float* x, const fl
 /* Features of every contact (original order, [nc][22]) and, if target is not
  * NULL, the scaled resident reaction ([nc][3]) as a training target. */
 FC3D_API int fc3d_hip_problem_features(fc3d_hip_problem* h, float* feat, float* target);
-/* The model used by warm_start = 2 on this problem (not owned; NULL detaches). */
+/* The model used by warm_start = 2, 3 on this problem (not owned; NULL detaches). */
 FC3D_API int fc3d_hip_problem_set_model(fc3d_hip_problem* h, fc3d_hip_mlp* m);
 /* The model fc3d_nsgs_hip attaches when iparam[WARM_START] = 2 (not owned). */
 FC3D_API void fc3d_hip_set_default_model(fc3d_hip_mlp* m);

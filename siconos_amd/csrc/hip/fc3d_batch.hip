@@ -502,7 +502,7 @@ int fc3d_hip_batch_solve(fc3d_hip_batch* b, const fc3d_hip_params* p, fc3d_hip_b
     set_error("fc3d_hip_batch_solve: local solver %d not supported", ls);
     return NUMERICS_ERR_INVALID_SOLVER;
   }
-  if (p->error_evaluation < 0 || p->error_evaluation > 3 || p->warm_start == 2) {
+  if (p->error_evaluation < 0 || p->error_evaluation > 3 || p->warm_start >= 2) {
     set_error("fc3d_hip_batch_solve: unsupported evaluation mode or warm start");
     return NUMERICS_ERR_INVALID_ARGUMENT;
   }

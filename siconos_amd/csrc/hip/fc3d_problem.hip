@@ -776,12 +776,14 @@ static int final_residual(fc3d_hip_problem* h) {
   return 0;
 }
 
-// Learned warm start with its fallback: one sweep of `phase` from the zero
-// start and one from the network's start, each followed by the FP64
-// residual; the solve continues from the state with the smaller residual
-// (the zero start on a tie or a non-finite residual).  The kept sweep counts
-// as the solve's first; the other one is extra work inside the solve's time.
-static int learned_start(fc3d_hip_problem* h, int phase, int schedule, int max_iter,
+// Learned warm start with its fallback: one sweep of `phase` from the
+// fallback start (zero, or with `previous` the resident reaction, i.e. the
+// previous time step's solution) and one from the network's start, each
+// followed by the FP64 residual; the solve continues from the state with the
+// smaller residual (the fallback on a tie or a non-finite residual).  The kept
+// sweep counts as the solve's first; the other one is extra work inside the
+// solve's time.
+static int learned_start(fc3d_hip_problem* h, int phase, int schedule, int max_iter, int previous,
                          SolveState* s0, fc3d_hip_stats* out) {
   const size_t ns = (size_t)h->nslots;
   const unsigned nb = (unsigned)((ns + 255) / 256);
@@ -792,13 +794,19 @@ static int learned_start(fc3d_hip_problem* h, int phase, int schedule, int max_i
         (rc = dev_alloc(&h->rbak, ns)) || (rc = dev_alloc(&h->plibak, ns)))
       return rc;
   }
+  if (previous)  // keep the resident reaction for the fallback trial
+    FC3D_HIP_TRY(hipMemcpyAsync(h->x, h->r, sizeof(double4) * ns, hipMemcpyDeviceToDevice,
+                                h->stream));
   hipLaunchKernelGGL(k_features, dim3(nb), dim3(256), 0, h->stream, ptrs64(h, h->part), h->nslots,
                      h->feat, (float*)nullptr);
   if ((rc = mlp_forward_device(h->model, h->nslots, h->feat, h->featX, h->ypred, h->stream)))
     return rc;
   double res[2];
   for (int trial = 0; trial < 2; trial++) {
-    if (trial == 0)
+    if (trial == 0 && previous)
+      FC3D_HIP_TRY(hipMemcpyAsync(h->r, h->x, sizeof(double4) * ns, hipMemcpyDeviceToDevice,
+                                  h->stream));
+    else if (trial == 0)
       hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)((4 * ns + 255) / 256)), dim3(256), 0,
                          h->stream, reinterpret_cast<double*>(h->r), 0.0, 4 * ns);
     else
@@ -836,8 +844,8 @@ static int learned_start(fc3d_hip_problem* h, int phase, int schedule, int max_i
     FC3D_HIP_TRY(hipMemcpyAsync(h->pli, h->plibak, sizeof(double) * ns, hipMemcpyDeviceToDevice,
                                 h->stream));
   }
-  out->warm_start_used = keep ? 2 : 1;
-  out->start_residual_zero = res[0];
+  out->warm_start_used = keep ? 2 : (previous ? 0 : 1);
+  out->start_residual_fallback = res[0];
   out->start_residual_learned = res[1];
   if (!s0->stop) {
     s0->iter = 1;
@@ -1214,8 +1222,8 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
     set_error("fc3d_hip_solve: rho strategy %d not supported", p->rho_strategy);
     return NUMERICS_ERR_NOT_IMPLEMENTED;
   }
-  if (p->warm_start < 0 || p->warm_start > 2) {
-    set_error("fc3d_hip_solve: warm_start %d is not 0, 1 or 2", p->warm_start);
+  if (p->warm_start < 0 || p->warm_start > 3) {
+    set_error("fc3d_hip_solve: warm_start %d is not 0, 1, 2 or 3", p->warm_start);
     return NUMERICS_ERR_INVALID_ARGUMENT;
   }
   if (p->accel_depth < 0 || p->accel_depth > AA_MAX || p->accel_every < 1 ||
@@ -1224,8 +1232,9 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
               p->accel_depth, AA_MAX, p->accel_every, p->accel_reg);
     return NUMERICS_ERR_INVALID_ARGUMENT;
   }
-  if (p->warm_start == 2 && !h->model) {
-    set_error("fc3d_hip_solve: warm_start 2 needs a model (fc3d_hip_problem_set_model)");
+  if (p->warm_start >= 2 && !h->model) {
+    set_error("fc3d_hip_solve: warm_start %d needs a model (fc3d_hip_problem_set_model)",
+              p->warm_start);
     return NUMERICS_ERR_INVALID_OPTION;
   }
   int rc = use_device(h->device);
@@ -1295,7 +1304,8 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
     for (auto& e : kev) FC3D_HIP_TRY(hipEventCreate(&e));
   }
   FC3D_HIP_TRY(hipEventRecord(t0, h->stream));
-  if (p->warm_start == 2 && (rc = learned_start(h, mixed, sched[mixed], p->max_iter, &s0, s)))
+  if (p->warm_start >= 2 &&
+      (rc = learned_start(h, mixed, sched[mixed], p->max_iter, p->warm_start == 3, &s0, s)))
     return rc;
   const int chunk = p->sweeps_per_launch > 0 ? p->sweeps_per_launch : 1;
   // FC3D_HIP_NO_GRAPH=1 enqueues the same kernels directly instead of

@@ -78,8 +78,8 @@ void fc3d_nsgs_hip(FrictionContactProblem* problem, double* reaction, double* ve
   fc3d_hip_problem* h = NULL;
   rc = fc3d_problem_to_device(problem, options->iparam[SICONOS_FRICTION_3D_NSGS_HIP_DEVICE], &h);
   if (!rc) rc = fc3d_hip_set_reaction(h, reaction);
-  /* warm start 2 uses the process-wide model (fc3d_hip_set_default_model) */
-  if (!rc && p.warm_start == 2) rc = fc3d_hip_problem_set_model(h, fc3d_hip_default_model());
+  /* warm starts 2 and 3 use the process-wide model (fc3d_hip_set_default_model) */
+  if (!rc && p.warm_start >= 2) rc = fc3d_hip_problem_set_model(h, fc3d_hip_default_model());
   double t1 = now_ms();
   fc3d_hip_stats s;
   if (!rc) rc = fc3d_hip_solve(h, &p, &s);

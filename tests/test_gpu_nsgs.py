@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 from conftest import load_fixture, natural_residual
+from test_oracle_nsgs import COLLECTION1
 from siconos_amd import capi
 from siconos_amd import numerics as N
 
@@ -69,11 +70,10 @@ def test_plain_fp64_bit_exact_with_oracle_in_colour_order(oracle, cfg):
 
 
 @pytest.mark.parametrize("variant", range(5))
-@pytest.mark.parametrize("name", ["Capsules-i122-1617.dat", "Capsules-i101-404.dat",
-                                  "NESpheres_10_1.dat", "Rover1039.dat", "Rover9770.dat",
-                                  "FrictionContact3D_1c.dat", "FC3D_Example1_SBM.dat",
-                                  "BoxesStack1-i100000-32.hdf5.dat", "Confeti-ex03-Fc3D-SBM.dat",
-                                  "OneObject-i1028-138.hdf5.dat", "Rover4396.dat"])
+@pytest.mark.parametrize("name", COLLECTION1 + ["BoxesStack1-i100000-32.hdf5.dat",
+                                                "Confeti-ex03-Fc3D-SBM.dat",
+                                                "Confeti-ex13-4contact-Fc3D-SBM.dat",
+                                                "OneObject-i1028-138.hdf5.dat", "Rover4396.dat"])
 def test_fixtures_through_fc3d_driver(oracle, name, variant):
     """fc3d_driver with SICONOS_FRICTION_3D_NSGS_HIP and the five option sets
     of the reference test/test_nsgs_1.c, against the oracle in colour order."""

@@ -173,9 +173,9 @@ def test_learned_warm_start_gains_over_the_zero_start(trained_cfg2_model):
         assert zero.info == 0 and zero.residual <= 1e-6
         assert learned.info == 0 and learned.residual <= 1e-6
         report.append((seed, zero.iterations, learned.iterations, learned.warm_start_used,
-                       learned.start_residual_zero, learned.start_residual_learned))
+                       learned.start_residual_fallback, learned.start_residual_learned))
         # the fallback guarantees the learned path is never worse after its first sweep
-        assert learned.start_residual_learned < learned.start_residual_zero or \
+        assert learned.start_residual_learned < learned.start_residual_fallback or \
             learned.warm_start_used == 1
     print("\n[ablation] train loss %.4f, training sweeps %s" % (loss, train_sweeps))
     for r in report:
@@ -207,7 +207,7 @@ def test_fallback_keeps_the_zero_start_bit_for_bit(lib):
     r0 = H.reaction()
     fb = H.solve(N.hip_params(tol=1e-8, max_iter=1000, warm_start=2))
     assert fb.warm_start_used == 1
-    assert fb.start_residual_learned > fb.start_residual_zero
+    assert fb.start_residual_learned > fb.start_residual_fallback
     assert fb.iterations == zero.iterations
     assert np.array_equal(H.reaction(), r0)
 

@@ -10,9 +10,15 @@ from siconos_amd import numerics as N
 
 # files of the reference's data_collection_1.c that are committed here; the
 # five option sets of test_nsgs_1.c are expected to converge on all of them
-COLLECTION1 = ["FC3D_Example1_SBM.dat", "FC3D_Example1.dat", "FrictionContact3D_1c.dat",
-               "FrictionContact3D_RR_1c.dat", "Capsules-i122-1617.dat", "Capsules-i101-404.dat",
-               "NESpheres_10_1.dat", "Rover1039.dat", "Rover11035.dat", "Rover9770.dat"]
+# the reference's data_collection_1.c, all 25 files (used by test_nsgs_1.c)
+COLLECTION1 = ["FC3D_Example1.dat", "FC3D_Example1_SBM.dat", "FrictionContact3D_1c.dat",
+               "FrictionContact3D_RR_1c.dat", "Capsules-i100-1090.dat", "Capsules-i100-889.dat",
+               "Capsules-i101-404.dat", "Capsules-i103-990.dat", "Capsules-i122-1617.dat",
+               "Rover1039.dat", "Rover1040.dat", "Rover1041.dat", "Rover11035.dat",
+               "Rover11211.dat", "Rover3865.dat", "Rover4144.dat", "Rover4493.dat",
+               "Rover4516.dat", "Rover4609.dat", "Rover4613.dat", "Rover4622.dat",
+               "Rover9770.dat", "NESpheres_10_1.dat", "NESpheres_30_1.dat",
+               "Confeti-ex13-Fc3D-SBM.dat"]
 
 
 def example1_solution():
