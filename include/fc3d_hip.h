@@ -115,6 +115,14 @@ FC3D_API int64_t fc3d_hip_problem_nnzb(const fc3d_hip_problem* h);
 FC3D_API int fc3d_hip_problem_colours(const fc3d_hip_problem* h, int* colour);
 /* sweep order: the original index of the contact at each sweep position */
 FC3D_API int fc3d_hip_problem_order(const fc3d_hip_problem* h, int* order);
+/* New values for an unchanged structure (a new time step of the same contact
+ * set): W's blocks in the CSR order given at creation, q and mu in the
+ * caller's order; any pointer may be NULL (kept).  The colouring, layout,
+ * tuned schedule and captured graphs stay valid, and so does the resident
+ * reaction, which warm_start = 0 or 3 then reuses (LinearOSNS.cpp:873-891
+ * keeps the previous step's reaction as the start in the same way). */
+FC3D_API int fc3d_hip_problem_update(fc3d_hip_problem* h, const double* blocks, const double* q,
+                                     const double* mu);
 FC3D_API int fc3d_hip_set_reaction(fc3d_hip_problem* h, const double* reaction);
 FC3D_API int fc3d_hip_get_reaction(const fc3d_hip_problem* h, double* reaction);
 FC3D_API int fc3d_hip_get_velocity(const fc3d_hip_problem* h, double* velocity);

@@ -165,6 +165,7 @@ SIGNATURES = {
     "fc3d_hip_problem_nnzb": (C.c_int64, [H_p]),
     "fc3d_hip_problem_colours": (C.c_int, [H_p, c_int_p]),
     "fc3d_hip_problem_order": (C.c_int, [H_p, c_int_p]),
+    "fc3d_hip_problem_update": (C.c_int, [H_p, c_double_p, c_double_p, c_double_p]),
     "fc3d_hip_set_reaction": (C.c_int, [H_p, c_double_p]),
     "fc3d_hip_get_reaction": (C.c_int, [H_p, c_double_p]),
     "fc3d_hip_get_velocity": (C.c_int, [H_p, c_double_p]),

@@ -52,7 +52,8 @@ int default_sort_window() {
 
 void append_problem(HostLayout& L, int c0, int n, const int64_t* row_ptr, const int64_t* col_idx,
                     const double* blocks, const double* q, const double* mu, const int* colour,
-                    const RowInfo& info, int window, int* ncolours_out) {
+                    const RowInfo& info, int window, int* ncolours_out,
+                    std::vector<int64_t>* block_dst) {
   int ncol = 0;
   for (int i = c0; i < c0 + n; i++) ncol = std::max(ncol, colour[i] + 1);
   *ncolours_out = ncol;
@@ -117,6 +118,7 @@ void append_problem(HostLayout& L, int c0, int n, const int64_t* row_ptr, const 
     for (int k = 0; k < L.kmax[(size_t)sl]; k++)
       for (int l = 0; l < 64; l++)
         L.col[((size_t)L.koff[(size_t)sl] + (size_t)k) * 64 + (size_t)l] = sl * 64 + l;
+  if (block_dst) block_dst->assign((size_t)(row_ptr[c0 + n] - row_ptr[c0]), 0);
   for (int slot = slice0 * 64; slot < L.nslots; slot++) {
     const int i = L.slot2orig[(size_t)slot];
     if (i < 0) continue;
@@ -130,9 +132,13 @@ void append_problem(HostLayout& L, int c0, int n, const int64_t* row_ptr, const 
       L.col[plane * 64 + (size_t)l] = L.orig2slot[(size_t)col_idx[k]];
       for (int e = 0; e < 9; e++)
         L.vals[(plane * 9 + (size_t)e) * 64 + (size_t)l] = blocks[9 * k + e];
+      if (block_dst)
+        (*block_dst)[(size_t)(k - row_ptr[c0])] = (int64_t)(plane * 9 * 64 + (size_t)l);
       kk++;
     }
     const int64_t d = info.diag_block[(size_t)i];
+    if (block_dst)
+      (*block_dst)[(size_t)(d - row_ptr[c0])] = -1 - (int64_t)((size_t)sl * 9 * 64 + (size_t)l);
     for (int e = 0; e < 9; e++)
       L.diag[((size_t)sl * 9 + (size_t)e) * 64 + (size_t)l] = blocks[9 * d + e];
     for (int e = 0; e < 3; e++) L.q[((size_t)sl * 3 + (size_t)e) * 64 + (size_t)l] = q[3 * i + e];

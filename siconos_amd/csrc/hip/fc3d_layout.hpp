@@ -32,9 +32,14 @@ struct HostLayout {
 int validate_rows(int c0, int n, const int64_t* row_ptr, const int64_t* col_idx, RowInfo* info,
                   std::string* err);
 int default_sort_window();
+// block_dst (optional, one entry per CSR block of the problem, indexed from
+// row_ptr[c0]): where block k's first value lands, as (plane * 9 * 64 + lane)
+// in `vals` for an off-diagonal block or as -1 - (slice * 9 * 64 + lane) in
+// `diag` for the diagonal one; value e then sits 64 entries further per e.
 void append_problem(HostLayout& L, int c0, int n, const int64_t* row_ptr, const int64_t* col_idx,
                     const double* blocks, const double* q, const double* mu, const int* colour,
-                    const RowInfo& info, int window, int* ncolours_out);
+                    const RowInfo& info, int window, int* ncolours_out,
+                    std::vector<int64_t>* block_dst = nullptr);
 
 // Dependencies of the single-launch (dataflow) sweep.  A slice may start once
 // every slice holding a neighbour of lower colour has finished the current

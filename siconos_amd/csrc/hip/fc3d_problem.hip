@@ -192,19 +192,43 @@ __global__ __launch_bounds__(256) void k_residual(SweepPtrs<double> P, const Sol
   }
 }
 
-// Fixed-order sum of `n` strided partials with one block of 1024 threads.
-__device__ double block_sum(const double* part, int n, int stride, int off, double* sh) {
-  double v = 0.0;
-  for (int i = threadIdx.x; i < n; i += 1024) v += part[(size_t)i * stride + off];
-  v = wave_sum(v);
+// Fixed-order sums of NV interleaved partials (part[i * NV + j], j < NV) with
+// one block of 1024 threads: thread t adds entries t, t + 1024, ... in that
+// order (eight loads in flight at a time), then waves and the 16 wave sums are
+// added in a fixed order.  Results are valid on thread 0.
+template <int NV>
+__device__ void block_sums(const double* __restrict__ part, int n, double* sh, double* out) {
+  double v[NV];
+#pragma unroll
+  for (int j = 0; j < NV; j++) v[j] = 0.0;
+  int i = threadIdx.x;
+  for (; i + 7 * 1024 < n; i += 8 * 1024) {
+    double t[8][NV];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+#pragma unroll
+      for (int j = 0; j < NV; j++) t[u][j] = part[(size_t)(i + u * 1024) * NV + j];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+#pragma unroll
+      for (int j = 0; j < NV; j++) v[j] += t[u][j];
+  }
+  for (; i < n; i += 1024)
+#pragma unroll
+    for (int j = 0; j < NV; j++) v[j] += part[(size_t)i * NV + j];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  __syncthreads();
-  if (lane == 0) sh[wave] = v;
-  __syncthreads();
-  double t = 0.0;
-  if (threadIdx.x == 0)
-    for (int w = 0; w < 16; w++) t += sh[w];
-  return t;  // valid on thread 0
+#pragma unroll
+  for (int j = 0; j < NV; j++) {
+    const double w = wave_sum(v[j]);
+    __syncthreads();
+    if (lane == 0) sh[wave] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int k = 0; k < 16; k++) t += sh[k];
+      out[j] = t;
+    }
+  }
 }
 
 // End of a sweep: light error (fc3d_nsgs.c:90-96) and the convergence test of
@@ -214,8 +238,9 @@ __global__ __launch_bounds__(1024) void k_finalize_sweep(const DevParams* prm, S
                                                          int want_phase) {
   __shared__ double sh[16];
   if (st->stop || st->phase != want_phase) return;
-  const double light = block_sum(part, nslices, 2, 0, sh);
-  const double nr2 = block_sum(part, nslices, 2, 1, sh);
+  double sums[2];
+  block_sums<2>(part, nslices, sh, sums);
+  const double light = sums[0], nr2 = sums[1];
   if (threadIdx.x != 0) return;
   st->iter += 1;
   st->nf = sqrt(light);
@@ -260,9 +285,9 @@ __global__ __launch_bounds__(1024) void k_finalize_residual(const DevParams* prm
                                                             int plain) {
   __shared__ double sh[16];
   if (!plain && !st->need_full) return;
-  const double e2 = block_sum(part, nslices, 3, 0, sh);
-  const double r2 = block_sum(part, nslices, 3, 1, sh);
-  const double u2 = block_sum(part, nslices, 3, 2, sh);
+  double sums[3];
+  block_sums<3>(part, nslices, sh, sums);
+  const double e2 = sums[0], r2 = sums[1], u2 = sums[2];
   if (threadIdx.x != 0) return;
   double err = sqrt(e2);
   const double scale = fmax(st->norm_q, fmax(sqrt(r2), sqrt(u2)));
@@ -388,6 +413,32 @@ __global__ void k_apply_start(SweepPtrs<double> P, int nslots, const float* __re
   P.r[slot] = make_double4(r[0], r[1], r[2], 0.0);
 }
 
+// In-place value update for an unchanged sparsity: CSR block k goes where the
+// layout put it (dst >= 0: vals plane entry, dst < 0: diagonal entry).
+__global__ void k_scatter_blocks(int64_t n9, const double* __restrict__ blocks,
+                                 const int64_t* __restrict__ dst, double* __restrict__ vals,
+                                 double* __restrict__ diag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n9) return;
+  const int64_t k = i / 9, e = i % 9;
+  const int64_t d = dst[k];
+  if (d >= 0)
+    vals[d + e * 64] = blocks[i];
+  else
+    diag[(-1 - d) + e * 64] = blocks[i];
+}
+
+__global__ void k_scatter_contacts(int nc, const double* __restrict__ q,
+                                   const double* __restrict__ mu, const int* __restrict__ o2s,
+                                   double* __restrict__ Q, double* __restrict__ MU) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  const int slot = o2s[i], s = slot >> 6, l = slot & 63;
+  if (q)
+    for (int e = 0; e < 3; e++) Q[(size_t)(s * 3 + e) * 64 + l] = q[3 * (size_t)i + e];
+  if (mu) MU[slot] = mu[i];
+}
+
 template <typename T>
 __global__ void k_fill(T* p, T v, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -470,6 +521,10 @@ struct fc3d_hip_problem {
   int df_grid = 0;
   int *dep_ptr = nullptr, *dep_idx = nullptr, *qlist = nullptr, *done = nullptr;
   DataflowState* df = nullptr;
+  // in-place updates: CSR block -> layout position, uploaded on first use
+  std::vector<int64_t> block_dst;
+  int64_t* d_block_dst = nullptr;
+  int* d_o2s = nullptr;
 };
 
 namespace fc3d {
@@ -1021,7 +1076,7 @@ int fc3d_hip_problem_create(int nc, const int64_t* row_ptr, const int64_t* col_i
   HostLayout L;
   int ncol = 0;
   append_problem(L, 0, nc, row_ptr, col_idx, blocks, q, mu, h->colour.data(), info,
-                 default_sort_window(), &ncol);
+                 default_sort_window(), &ncol, &h->block_dst);
   if (L.nplanes * 64 > 0x7fffffffLL) {
     set_error("fc3d_hip_problem_create: sliced layout exceeds int32 indexing");
     return fail(NUMERICS_ERR_INVALID_ARGUMENT);
@@ -1068,7 +1123,7 @@ void fc3d_hip_problem_destroy(fc3d_hip_problem* h) {
   for (void* p : ptrs)
     if (p) hipFree(p);
   free_accel(h);
-  void* dfp[] = {h->dep_ptr, h->dep_idx, h->qlist, h->done, h->df};
+  void* dfp[] = {h->dep_ptr, h->dep_idx, h->qlist, h->done, h->df, h->d_block_dst, h->d_o2s};
   for (void* p : dfp)
     if (p) hipFree(p);
   if (h->stream) hipStreamDestroy(h->stream);
@@ -1160,6 +1215,84 @@ int fc3d_hip_gemv(fc3d_hip_problem* h, const double* x, double* y) {
   FC3D_HIP_TRY(hipGetLastError());
   FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
   return get_vec(h, h->u, y);
+}
+
+int fc3d_hip_problem_update(fc3d_hip_problem* h, const double* blocks, const double* q,
+                            const double* mu) {
+  if (!h) return NUMERICS_ERR_NULL_POINTER;
+  int rc = use_device(h->device);
+  if (rc) return rc;
+  const size_t ns = (size_t)h->nslots;
+  if (blocks && h->nnzb > 0) {
+    if (!h->d_block_dst && (rc = dev_upload(&h->d_block_dst, h->block_dst))) return rc;
+    double* d = nullptr;
+    if ((rc = dev_alloc(&d, (size_t)h->nnzb * 9))) return rc;
+    hipError_t e = hipMemcpyAsync(d, blocks, sizeof(double) * 9 * (size_t)h->nnzb,
+                                  hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) {
+      const int64_t n9 = 9 * h->nnzb;
+      hipLaunchKernelGGL(k_scatter_blocks, dim3((unsigned)((n9 + 255) / 256)), dim3(256), 0,
+                         h->stream, n9, (const double*)d, (const int64_t*)h->d_block_dst, h->vals,
+                         h->diag);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    hipFree(d);
+    if (e != hipSuccess) {
+      set_error("fc3d_hip_problem_update: %s", hipGetErrorString(e));
+      return NUMERICS_ERR_DEVICE;
+    }
+  }
+  if ((q || mu) && h->nc > 0) {
+    if (!h->d_o2s && (rc = dev_upload(&h->d_o2s, h->orig2slot))) return rc;
+    double *dq = nullptr, *dmu = nullptr;
+    if ((q && (rc = dev_alloc(&dq, 3 * (size_t)h->nc))) ||
+        (mu && (rc = dev_alloc(&dmu, (size_t)h->nc)))) {
+      hipFree(dq);
+      return rc;
+    }
+    hipError_t e = hipSuccess;
+    if (q) e = hipMemcpyAsync(dq, q, sizeof(double) * 3 * (size_t)h->nc, hipMemcpyHostToDevice,
+                              h->stream);
+    if (e == hipSuccess && mu)
+      e = hipMemcpyAsync(dmu, mu, sizeof(double) * (size_t)h->nc, hipMemcpyHostToDevice,
+                         h->stream);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_scatter_contacts, dim3((unsigned)((h->nc + 255) / 256)), dim3(256), 0,
+                         h->stream, h->nc, (const double*)dq, (const double*)dmu,
+                         (const int*)h->d_o2s, h->q, h->mu);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    hipFree(dq);
+    hipFree(dmu);
+    if (e != hipSuccess) {
+      set_error("fc3d_hip_problem_update: %s", hipGetErrorString(e));
+      return NUMERICS_ERR_DEVICE;
+    }
+    if (q) {
+      // |q| in the caller's order, as dnrm2 does (fc3d_nsgs.c:571)
+      double s2 = 0.0;
+      for (int64_t i = 0; i < 3 * (int64_t)h->nc; i++) s2 += q[i] * q[i];
+      h->norm_q = std::sqrt(s2);
+    }
+  }
+  // FP32 copies follow the FP64 values
+  if (h->vals32) {
+    auto conv = [&](const double* a, float* b, size_t n) {
+      if (n) hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((n + 255) / 256)),
+                                dim3(256), 0, h->stream, a, b, n);
+    };
+    if (blocks) {
+      conv(h->vals, h->vals32, (size_t)h->nplanes * 9 * 64);
+      conv(h->diag, h->diag32, (size_t)h->nslices * 9 * 64);
+    }
+    if (q) conv(h->q, h->q32, (size_t)h->nslices * 3 * 64);
+    if (mu) conv(h->mu, h->mu32, ns);
+    FC3D_HIP_TRY(hipGetLastError());
+    FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  return 0;
 }
 
 int fc3d_hip_problem_features(fc3d_hip_problem* h, float* feat, float* target) {

@@ -321,6 +321,13 @@ class HipProblem:
         self._check(self._lib.fc3d_hip_problem_order(self._h, iptr(o)), "order")
         return o
 
+    def update(self, values=None, q=None, mu=None):
+        """New W block values (same CSR order and structure), q and/or mu."""
+        a = [None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+             for v in (values, q, mu)]
+        self._check(self._lib.fc3d_hip_problem_update(
+            self._h, *[None if v is None else dptr(v) for v in a]), "update")
+
     def set_reaction(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64)
         self._check(self._lib.fc3d_hip_set_reaction(self._h, dptr(r)), "set_reaction")
