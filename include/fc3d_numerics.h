@@ -414,6 +414,10 @@ typedef struct {
   double* mu;        /* nc                                                */
   int* body_a;       /* nc: first body of each contact                    */
   int* body_b;       /* nc: second body                                   */
+  double* H;         /* nc * 36: per contact the 3x6 blocks for body_a and
+                        body_b, column-major (generators only)             */
+  double* minv;      /* nbodies * 6: diagonal of M_k^-1 (generators only)  */
+  double* vfree;     /* nbodies * 6 (generators only)                     */
 } fc3d_generated;
 
 /* Spheres r=1, m=1, I=0.4; random distinct body pairs; random unit normals;
@@ -429,6 +433,23 @@ FC3D_API void fc3d_generated_free(fc3d_generated* g);
 /* Wraps a generated problem as a FrictionContactProblem with SBM storage
  * (copies the data). */
 FC3D_API FrictionContactProblem* fc3d_generated_to_problem(const fc3d_generated* g);
+
+/* W = H M^-1 H^T and q = H v_free assembled on the GPU: the interaction block
+ * of contacts c and d is the sum, over the bodies k they share, of
+ * H_{c,k} M_k^-1 H_{d,k}^T (LinearOSNS.cpp:522-700, numerics
+ * GlobalFrictionContactProblem.c:399-446 for the same product on SBM
+ * matrices), stored as block CSR like BlockCSRMatrix.cpp:234-263 builds it.
+ * Contact c touches bodies body_a[c] and body_b[c] (-1: a fixed body with no
+ * degrees of freedom) through the 3x6 column-major blocks H + 36 c and
+ * H + 36 c + 18.  Minv holds per body a 6x6 column-major block, or its
+ * diagonal (6 values) when minv_diagonal != 0.  On return `out` holds nc,
+ * nbodies, nnzb, row_ptr, col_idx (ascending, diagonal present), blocks and
+ * q; the other pointers are NULL.  Release with fc3d_generated_free.
+ * device_ms (may be NULL) receives the device time of the numeric kernels. */
+FC3D_API int fc3d_hip_assemble(int nbodies, int nc, const int* body_a, const int* body_b,
+                               const double* H, const double* Minv, int minv_diagonal,
+                               const double* vfree, int device, fc3d_generated* out,
+                               double* device_ms);
 
 /* ------------------------------------------------------------------------ */
 /* Colouring (no reference counterpart; replaces PETSc MatColoring GREEDY of

@@ -96,7 +96,8 @@ class HipBatchStats(C.Structure):
 class Generated(C.Structure):
     _fields_ = [("nc", C.c_int), ("nbodies", C.c_int), ("nnzb", C.c_int64),
                 ("row_ptr", c_int64_p), ("col_idx", c_int64_p), ("blocks", c_double_p),
-                ("q", c_double_p), ("mu", c_double_p), ("body_a", c_int_p), ("body_b", c_int_p)]
+                ("q", c_double_p), ("mu", c_double_p), ("body_a", c_int_p), ("body_b", c_int_p),
+                ("H", c_double_p), ("minv", c_double_p), ("vfree", c_double_p)]
 
 
 FCP_p = C.POINTER(FrictionContactProblem)
@@ -152,6 +153,9 @@ SIGNATURES = {
                                               C.c_uint64, C.POINTER(Generated)]),
     "fc3d_generated_free": (None, [C.POINTER(Generated)]),
     "fc3d_generated_to_problem": (FCP_p, [C.POINTER(Generated)]),
+    "fc3d_hip_assemble": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p, c_double_p, c_double_p,
+                                    C.c_int, c_double_p, C.c_int, C.POINTER(Generated),
+                                    c_double_p]),
     "fc3d_colour_contacts": (C.c_int, [C.c_int, c_int64_p, c_int64_p, c_int_p]),
     "fc3d_colouring_is_valid": (C.c_int, [C.c_int, c_int64_p, c_int64_p, c_int_p]),
     # fc3d_hip.h

@@ -160,6 +160,23 @@ static int assemble(int nb, int nc, const int* A, const int* B, const double* no
   memcpy(out->body_a, A, sizeof(int) * (size_t)nc);
   memcpy(out->body_b, B, sizeof(int) * (size_t)nc);
   memcpy(out->mu, mu, sizeof(double) * (size_t)nc);
+  /* the inputs of the assembly, for callers that assemble elsewhere */
+  out->H = (double*)malloc(sizeof(double) * 36 * (size_t)nc);
+  out->minv = (double*)malloc(sizeof(double) * 6 * (size_t)nb);
+  out->vfree = (double*)malloc(sizeof(double) * 6 * (size_t)nb);
+  if (!out->H || !out->minv || !out->vfree) return NUMERICS_ERR_MEMORY;
+  for (int c = 0; c < nc; c++)
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        double* h = out->H + 36 * (size_t)c;
+        h[i + 3 * j] = J[c].Ta[3 * i + j];
+        h[i + 3 * (j + 3)] = J[c].Ra[3 * i + j];
+        h[18 + i + 3 * j] = J[c].Tb[3 * i + j];
+        h[18 + i + 3 * (j + 3)] = J[c].Rb[3 * i + j];
+      }
+  for (int k = 0; k < nb; k++)
+    for (int j = 0; j < 6; j++) out->minv[6 * (size_t)k + j] = j < 3 ? minv_t : minv_r;
+  memcpy(out->vfree, vfree, sizeof(double) * 6 * (size_t)nb);
   for (int c = 0; c < nc; c++) {
     int n = 0;
     for (int t = inc.ptr[A[c]]; t < inc.ptr[A[c] + 1]; t++) tmp[n++] = inc.list[t];
@@ -342,6 +359,7 @@ void fc3d_generated_free(fc3d_generated* g) {
   if (!g) return;
   free(g->row_ptr); free(g->col_idx); free(g->blocks); free(g->q); free(g->mu);
   free(g->body_a); free(g->body_b);
+  free(g->H); free(g->minv); free(g->vfree);
   memset(g, 0, sizeof *g);
 }
 

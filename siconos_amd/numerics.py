@@ -238,6 +238,13 @@ class GeneratedProblem:
         self.mu = np.ctypeslib.as_array(g.mu, shape=(n,)).copy()
         self.body_a = np.ctypeslib.as_array(g.body_a, shape=(n,)).copy()
         self.body_b = np.ctypeslib.as_array(g.body_b, shape=(n,)).copy()
+        # the inputs of the assembly: per contact the 3x6 blocks of both bodies
+        # (column-major), per body the diagonal of M^-1 and the free velocity
+        self.H = np.ctypeslib.as_array(g.H, shape=(n * 36,)).copy() if g.H else None
+        self.minv = (np.ctypeslib.as_array(g.minv, shape=(g.nbodies * 6,)).copy()
+                     if g.minv else None)
+        self.vfree = (np.ctypeslib.as_array(g.vfree, shape=(g.nbodies * 6,)).copy()
+                      if g.vfree else None)
 
     def problem(self):
         return FrictionContactProblem(3, self.W, self.q, self.mu)
@@ -261,6 +268,33 @@ def generate_random_pairs(nbodies, nc, mu_lo, mu_hi, seed):
 
 def generate_granular_pile(nbodies, nc, mu_lo, mu_hi, seed):
     return _generate("fc3d_generate_granular_pile", nbodies, nc, mu_lo, mu_hi, seed)
+
+
+def assemble(nbodies, body_a, body_b, H, minv, vfree, minv_diagonal=True, device=0):
+    """W = H M^-1 H^T and q = H v_free on the GPU (fc3d_hip_assemble).
+    Returns (SparseBlock3 W, q, device milliseconds)."""
+    lib = capi.load()
+    body_a = np.ascontiguousarray(body_a, dtype=np.int32)
+    body_b = np.ascontiguousarray(body_b, dtype=np.int32)
+    H = np.ascontiguousarray(H, dtype=np.float64)
+    minv = np.ascontiguousarray(minv, dtype=np.float64)
+    vfree = np.ascontiguousarray(vfree, dtype=np.float64)
+    nc = len(body_a)
+    g = capi.Generated()
+    ms = C.c_double()
+    rc = lib.fc3d_hip_assemble(nbodies, nc, iptr(body_a), iptr(body_b), dptr(H), dptr(minv),
+                               int(minv_diagonal), dptr(vfree), device, C.byref(g), C.byref(ms))
+    if rc:
+        raise RuntimeError(f"fc3d_hip_assemble: {rc} {capi.last_error()}")
+    try:
+        nb = g.nnzb
+        W = SparseBlock3(np.ctypeslib.as_array(g.row_ptr, shape=(nc + 1,)).copy(),
+                         np.ctypeslib.as_array(g.col_idx, shape=(nb,)).copy(),
+                         np.ctypeslib.as_array(g.blocks, shape=(nb * 9,)).copy())
+        q = np.ctypeslib.as_array(g.q, shape=(3 * nc,)).copy()
+    finally:
+        lib.fc3d_generated_free(C.byref(g))
+    return W, q, ms.value
 
 
 def colour_contacts(W):
