@@ -69,6 +69,11 @@ typedef struct fc3d_hip_params {
   int accel_depth;           /* 0: off (default); 1..8 history columns           */
   int accel_every;           /* Anderson step on every n-th sweep (default 2)     */
   double accel_reg;          /* Tikhonov weight relative to the Gram trace (1e-10)*/
+  /* Sweeps in the velocity form (needs fc3d_hip_problem_set_bodies): the
+   * local problem from u = H v with body velocities v = v_free + M^-1 H^T r
+   * kept current, instead of the row of W; the same iteration in exact
+   * arithmetic.  Convergence is still judged on the FP64 residual with W. */
+  int velocity_form;         /* 0: rows of W (default); 1: velocity form          */
 } fc3d_hip_params;
 
 typedef struct fc3d_hip_stats {
@@ -117,12 +122,21 @@ FC3D_API int fc3d_hip_problem_colours(const fc3d_hip_problem* h, int* colour);
 FC3D_API int fc3d_hip_problem_order(const fc3d_hip_problem* h, int* order);
 /* New values for an unchanged structure (a new time step of the same contact
  * set): W's blocks in the CSR order given at creation, q and mu in the
- * caller's order; any pointer may be NULL (kept).  The colouring, layout,
+ * caller's order; any pointer may be NULL (kept).  New blocks or q drop
+ * factors attached with fc3d_hip_problem_set_bodies.  The colouring, layout,
  * tuned schedule and captured graphs stay valid, and so does the resident
  * reaction, which warm_start = 0 or 3 then reuses (LinearOSNS.cpp:873-891
  * keeps the previous step's reaction as the start in the same way). */
 FC3D_API int fc3d_hip_problem_update(fc3d_hip_problem* h, const double* blocks, const double* q,
                                      const double* mu);
+/* The factors of W = H M^-1 H^T for velocity-form sweeps (no reference
+ * counterpart; the factors are the inputs of fc3d_hip_assemble, in the
+ * caller's contact order).  Refused when the factors do not reproduce the
+ * resident diagonal blocks of W, or when two contacts of one body share a
+ * colour (W must hold a block for every pair of contacts sharing a body). */
+FC3D_API int fc3d_hip_problem_set_bodies(fc3d_hip_problem* h, int nbodies, const int* body_a,
+                                         const int* body_b, const double* H, const double* Minv,
+                                         int minv_diagonal, const double* vfree);
 FC3D_API int fc3d_hip_set_reaction(fc3d_hip_problem* h, const double* reaction);
 FC3D_API int fc3d_hip_get_reaction(const fc3d_hip_problem* h, double* reaction);
 FC3D_API int fc3d_hip_get_velocity(const fc3d_hip_problem* h, double* velocity);

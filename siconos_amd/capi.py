@@ -74,7 +74,8 @@ class HipParams(C.Structure):
                 ("hybrid_max_iter", C.c_int), ("precision", C.c_int), ("warm_start", C.c_int),
                 ("schedule", C.c_int), ("sweeps_per_launch", C.c_int), ("fp32_tol", C.c_double),
                 ("fp32_local_tol", C.c_double), ("instrument", C.c_int),
-                ("accel_depth", C.c_int), ("accel_every", C.c_int), ("accel_reg", C.c_double)]
+                ("accel_depth", C.c_int), ("accel_every", C.c_int), ("accel_reg", C.c_double),
+                ("velocity_form", C.c_int)]
 
 
 class HipStats(C.Structure):
@@ -170,6 +171,8 @@ SIGNATURES = {
     "fc3d_hip_problem_colours": (C.c_int, [H_p, c_int_p]),
     "fc3d_hip_problem_order": (C.c_int, [H_p, c_int_p]),
     "fc3d_hip_problem_update": (C.c_int, [H_p, c_double_p, c_double_p, c_double_p]),
+    "fc3d_hip_problem_set_bodies": (C.c_int, [H_p, C.c_int, c_int_p, c_int_p, c_double_p,
+                                              c_double_p, C.c_int, c_double_p]),
     "fc3d_hip_set_reaction": (C.c_int, [H_p, c_double_p]),
     "fc3d_hip_get_reaction": (C.c_int, [H_p, c_double_p]),
     "fc3d_hip_get_velocity": (C.c_int, [H_p, c_double_p]),

@@ -26,6 +26,7 @@
 
 #include "fc3d_internal.hpp"
 #include "fc3d_accel.hpp"
+#include "fc3d_velocity.hpp"
 #include "fc3d_layout.hpp"
 #include "fc3d_mlp.hpp"
 #include "fc3d_sweep.hpp"
@@ -49,6 +50,49 @@ __global__ __launch_bounds__(256, MINW) void k_sweep(SweepPtrs<T> P,
   const T tol = T(want_phase ? prm->fp32_local_tol : st->local_tol);
   double light, nr2;
   sweep_slot<T, U, PF>(P, prm, s, lane, tol, &light, &nr2);
+  light = wave_sum(light);
+  nr2 = wave_sum(nr2);
+  if (lane == 0) {
+    P.part[2 * (size_t)s] = light;
+    P.part[2 * (size_t)s + 1] = nr2;
+  }
+}
+
+// One colour of one sweep with four waves per slice: wave w gathers planes
+// w, w + 4, ... of the slice's 64 contacts, waves 1-3 hand their partial sums
+// to wave 0 through LDS, and wave 0 finishes the contacts (local solves,
+// write-back).  Four times the waves of k_sweep are in flight during the
+// gather, at no extra local-solve work.  The off-diagonal sum is taken in
+// another order than the reference row order, so this kernel serves only
+// the FP32 sweeps.
+template <typename T, int U>
+__global__ __launch_bounds__(256, 2) void k_sweep_w4(SweepPtrs<T> P,
+                                                     const DevParams* __restrict__ prm,
+                                                     const SolveState* __restrict__ st,
+                                                     int s_begin, int s_end, int want_phase) {
+  __shared__ T shy[3][3][64];
+  if (st->stop || st->phase != want_phase) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s = s_begin + xcd_block();
+  if (s >= s_end) return;
+  const int len = P.len[s * 64 + lane];
+  T y[3] = {T(0), T(0), T(0)};
+  gather_planes<T, U>(P, s, lane, len, wave, 4, y);
+  if (wave > 0)
+#pragma unroll
+    for (int e = 0; e < 3; e++) shy[wave - 1][e][lane] = y[e];
+  __syncthreads();
+  if (wave != 0) return;
+  double light = 0.0, nr2 = 0.0;
+  if (len >= 0) {
+    SlotInputs<T> in;
+    load_slot<T>(P, s, lane, &in);
+#pragma unroll
+    for (int e = 0; e < 3; e++)
+      in.ql[e] += ((y[e] + shy[0][e][lane]) + shy[1][e][lane]) + shy[2][e][lane];
+    const T tol = T(want_phase ? prm->fp32_local_tol : st->local_tol);
+    finish_slot<T>(P, prm, s, lane, tol, in, &light, &nr2);
+  }
   light = wave_sum(light);
   nr2 = wave_sum(nr2);
   if (lane == 0) {
@@ -525,6 +569,14 @@ struct fc3d_hip_problem {
   std::vector<int64_t> block_dst;
   int64_t* d_block_dst = nullptr;
   int* d_o2s = nullptr;
+  // velocity form (fc3d_hip_problem_set_bodies)
+  int nbodies = 0, minv_diag = 1, vel_on = 0, graph_vel[2] = {0, 0};
+  int *d_ba = nullptr, *d_bb = nullptr, *d_inc_ptr = nullptr, *d_inc = nullptr;
+  double *d_H = nullptr, *d_minv = nullptr, *d_vfree = nullptr;
+  double4* d_V = nullptr;
+  float *d_H32 = nullptr, *d_minv32 = nullptr;
+  float4* d_V32 = nullptr;
+  std::vector<double> vel_bytes64, vel_bytes32;  // per colour
 };
 
 namespace fc3d {
@@ -557,6 +609,20 @@ static SweepPtrs<float> ptrs32(fc3d_hip_problem* h) {
   return P;
 }
 
+static BodyPtrs<double> bodies64(fc3d_hip_problem* h) {
+  BodyPtrs<double> B;
+  B.ba = h->d_ba; B.bb = h->d_bb; B.H = h->d_H; B.minv = h->d_minv; B.minv_diag = h->minv_diag;
+  B.V = h->d_V; B.vfree = h->d_vfree; B.inc_ptr = h->d_inc_ptr; B.inc = h->d_inc;
+  return B;
+}
+static BodyPtrs<float> bodies32(fc3d_hip_problem* h) {
+  BodyPtrs<float> B;
+  B.ba = h->d_ba; B.bb = h->d_bb; B.H = h->d_H32; B.minv = h->d_minv32;
+  B.minv_diag = h->minv_diag; B.V = h->d_V32; B.vfree = h->d_vfree; B.inc_ptr = h->d_inc_ptr;
+  B.inc = h->d_inc;
+  return B;
+}
+
 static int ensure_fp32(fc3d_hip_problem* h) {
   if (h->vals32) return 0;
   const size_t nv = (size_t)h->nplanes * 9 * 64, nd = (size_t)h->nslices * 9 * 64,
@@ -581,11 +647,13 @@ static int ensure_fp32(fc3d_hip_problem* h) {
 
 // Enqueues one sweep of `phase` (1 FP32, 0 FP64) plus its convergence step.
 // Sweep-kernel schedules: unroll U, column prefetch PF, occupancy target.
-#define FC3D_NUM_SCHEDULES 9
+#define FC3D_NUM_SCHEDULES 10
 #define FC3D_SCHEDULE_DATAFLOW 7
+#define FC3D_SCHEDULE_W4 10
 static const char* const kScheduleNames[FC3D_NUM_SCHEDULES + 1] = {
     "auto", "u4/u8 occ2", "u4/u8 occ3", "u4/u8 occ4", "u4/u8 pf16 occ2", "u4/u8 pf16 occ3",
-    "u2/u4 pf8 occ4", "single launch, u4/u8 occ2", "u8/u16 occ1", "u8/u16 pf16 occ1"};
+    "u2/u4 pf8 occ4", "single launch, u4/u8 occ2", "u8/u16 occ1", "u8/u16 pf16 occ1",
+    "fp32: four waves per slice, u4"};
 
 // With events, the launch goes through hipExtLaunchKernelGGL, which stamps
 // them with the dispatch's own start and end (the times a kernel-trace
@@ -614,8 +682,24 @@ static void launch_sweep_t(fc3d_hip_problem* h, int phase, int grid, int sb, int
   }
 }
 
+static void launch_sweep_w4(fc3d_hip_problem* h, int sb, int se, hipEvent_t e0, hipEvent_t e1) {
+  auto k = k_sweep_w4<float, 4>;
+  SweepPtrs<float> P = ptrs32(h);
+  if (h->accel_on) P.f = h->aa32.f;
+  const dim3 g((unsigned)(se - sb)), b(256);
+  if (e0)
+    hipExtLaunchKernelGGL(k, g, b, 0, h->stream, e0, e1, 0, P, h->prm, (const SolveState*)h->st,
+                          sb, se, 1);
+  else
+    hipLaunchKernelGGL(k, g, b, 0, h->stream, P, h->prm, h->st, sb, se, 1);
+}
+
 static void launch_sweep(fc3d_hip_problem* h, int phase, int schedule, int grid, int sb, int se,
                          hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+  if (schedule == FC3D_SCHEDULE_W4 && phase) {
+    launch_sweep_w4(h, sb, se, e0, e1);
+    return;
+  }
   switch (schedule) {
     case 2: launch_sweep_t<4, 8, 0, 3>(h, phase, grid, sb, se, e0, e1); break;
     case 3: launch_sweep_t<4, 8, 0, 4>(h, phase, grid, sb, se, e0, e1); break;
@@ -624,8 +708,44 @@ static void launch_sweep(fc3d_hip_problem* h, int phase, int schedule, int grid,
     case 6: launch_sweep_t<2, 4, 8, 4>(h, phase, grid, sb, se, e0, e1); break;
     case 8: launch_sweep_t<8, 16, 0, 1>(h, phase, grid, sb, se, e0, e1); break;
     case 9: launch_sweep_t<8, 16, 16, 1>(h, phase, grid, sb, se, e0, e1); break;
-    default: launch_sweep_t<4, 8, 0, 2>(h, phase, grid, sb, se, e0, e1); break;  // 1 and 7
+    default: launch_sweep_t<4, 8, 0, 2>(h, phase, grid, sb, se, e0, e1); break;  // 1, 7, 10
   }
+}
+
+static void launch_sweep_vel(fc3d_hip_problem* h, int phase, int sb, int se, hipEvent_t e0,
+                             hipEvent_t e1) {
+  const dim3 g((unsigned)((se - sb + 3) / 4)), b(256);
+  if (phase) {
+    auto k = k_sweep_vel<float>;
+    SweepPtrs<float> P = ptrs32(h);
+    if (h->accel_on) P.f = h->aa32.f;
+    if (e0)
+      hipExtLaunchKernelGGL(k, g, b, 0, h->stream, e0, e1, 0, P, bodies32(h), h->prm,
+                            (const SolveState*)h->st, sb, se, 1);
+    else
+      hipLaunchKernelGGL(k, g, b, 0, h->stream, P, bodies32(h), h->prm, h->st, sb, se, 1);
+  } else {
+    auto k = k_sweep_vel<double>;
+    SweepPtrs<double> P = ptrs64(h, h->part);
+    if (h->accel_on) P.f = h->aa64.f;
+    if (e0)
+      hipExtLaunchKernelGGL(k, g, b, 0, h->stream, e0, e1, 0, P, bodies64(h), h->prm,
+                            (const SolveState*)h->st, sb, se, 0);
+    else
+      hipLaunchKernelGGL(k, g, b, 0, h->stream, P, bodies64(h), h->prm, h->st, sb, se, 0);
+  }
+}
+
+// Body velocities from the phase's reaction (st / as as in k_vinit).
+static void launch_vinit(fc3d_hip_problem* h, int phase, const SolveState* st,
+                         const AccelState* as) {
+  const dim3 g((unsigned)((h->nbodies + 255) / 256)), b(256);
+  if (phase)
+    hipLaunchKernelGGL(k_vinit<float>, g, b, 0, h->stream, bodies32(h), (const float4*)h->r32,
+                       h->nbodies, st, as, 1);
+  else
+    hipLaunchKernelGGL(k_vinit<double>, g, b, 0, h->stream, bodies64(h), (const double4*)h->r,
+                       h->nbodies, st, as, 0);
 }
 
 static DFArgs df_args(fc3d_hip_problem* h) {
@@ -666,9 +786,10 @@ static void launch_sweep_df(fc3d_hip_problem* h, int phase, hipEvent_t e0, hipEv
 
 // Event pairs one timed sweep uses, and the algorithmic bytes of pair c.
 static int timed_launches(const fc3d_hip_problem* h, int schedule) {
-  return schedule == FC3D_SCHEDULE_DATAFLOW ? 1 : h->ncolours;
+  return schedule == FC3D_SCHEDULE_DATAFLOW && !h->vel_on ? 1 : h->ncolours;
 }
 static double timed_bytes(const fc3d_hip_problem* h, int phase, int schedule, int c) {
+  if (h->vel_on) return phase ? h->vel_bytes32[(size_t)c] : h->vel_bytes64[(size_t)c];
   const std::vector<double>& b = phase ? h->colour_bytes32 : h->colour_bytes64;
   if (schedule != FC3D_SCHEDULE_DATAFLOW) return b[(size_t)c];
   double t = 0.0;
@@ -678,7 +799,13 @@ static double timed_bytes(const fc3d_hip_problem* h, int phase, int schedule, in
 
 static void enqueue_sweep(fc3d_hip_problem* h, int phase, int schedule, hipEvent_t* ev,
                           int nev) {
-  if (schedule == FC3D_SCHEDULE_DATAFLOW) {
+  if (h->vel_on) {
+    for (int c = 0; c < h->ncolours; c++) {
+      const int sb = h->colour_slice_ptr[(size_t)c], se = h->colour_slice_ptr[(size_t)c + 1];
+      const bool t = ev && 2 * c + 1 < nev;
+      launch_sweep_vel(h, phase, sb, se, t ? ev[2 * c] : nullptr, t ? ev[2 * c + 1] : nullptr);
+    }
+  } else if (schedule == FC3D_SCHEDULE_DATAFLOW) {
     launch_sweep_df(h, phase, ev && nev >= 2 ? ev[0] : nullptr, ev && nev >= 2 ? ev[1] : nullptr);
   } else {
     for (int c = 0; c < h->ncolours; c++) {
@@ -715,6 +842,8 @@ static void enqueue_sweep(fc3d_hip_problem* h, int phase, int schedule, hipEvent
       hipLaunchKernelGGL(k_aa_apply<double>, g, b, 0, h->stream, h->r, h->aa64, h->st, h->ast,
                          h->prm, h->nslots, 0);
     }
+    // an Anderson step moved r: the body velocities follow
+    if (h->vel_on) launch_vinit(h, phase, h->st, h->ast);
   }
 }
 
@@ -741,6 +870,24 @@ static int ensure_accel(fc3d_hip_problem* h, int mixed) {
   if (!h->ast && (rc = dev_alloc(&h->ast, 1))) return rc;
   if ((rc = alloc_accel(h, &h->aa64))) return rc;
   if (mixed && (rc = alloc_accel(h, &h->aa32))) return rc;
+  return 0;
+}
+
+// FP32 copies of the velocity-form factors.
+static int ensure_vel32(fc3d_hip_problem* h) {
+  if (h->d_H32) return 0;
+  const size_t nh = (size_t)h->nslices * 36 * 64;
+  const size_t nm = (size_t)h->nbodies * (h->minv_diag ? 6 : 36);
+  int rc;
+  if ((rc = dev_alloc(&h->d_H32, nh)) || (rc = dev_alloc(&h->d_minv32, nm)) ||
+      (rc = dev_alloc(&h->d_V32, 2 * (size_t)h->nbodies)))
+    return rc;
+  hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((nh + 255) / 256)), dim3(256), 0,
+                     h->stream, h->d_H, h->d_H32, nh);
+  hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((nm + 255) / 256)), dim3(256), 0,
+                     h->stream, h->d_minv, h->d_minv32, nm);
+  FC3D_HIP_TRY(hipGetLastError());
+  FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
   return 0;
 }
 
@@ -804,7 +951,7 @@ static int reset_accel(fc3d_hip_problem* h) {
 
 static int build_graph(fc3d_hip_problem* h, int phase, int sweeps, int schedule) {
   if (h->graph[phase] && h->graph_sweeps[phase] == sweeps && h->graph_schedule[phase] == schedule &&
-      h->graph_accel[phase] == h->accel_on)
+      h->graph_accel[phase] == h->accel_on && h->graph_vel[phase] == h->vel_on)
     return 0;
   if (h->graph[phase]) {
     hipGraphExecDestroy(h->graph[phase]);
@@ -819,6 +966,7 @@ static int build_graph(fc3d_hip_problem* h, int phase, int sweeps, int schedule)
   h->graph_sweeps[phase] = sweeps;
   h->graph_schedule[phase] = schedule;
   h->graph_accel[phase] = h->accel_on;
+  h->graph_vel[phase] = h->vel_on;
   return 0;
 }
 
@@ -1123,7 +1271,9 @@ void fc3d_hip_problem_destroy(fc3d_hip_problem* h) {
   for (void* p : ptrs)
     if (p) hipFree(p);
   free_accel(h);
-  void* dfp[] = {h->dep_ptr, h->dep_idx, h->qlist, h->done, h->df, h->d_block_dst, h->d_o2s};
+  void* dfp[] = {h->dep_ptr, h->dep_idx, h->qlist, h->done,    h->df,    h->d_block_dst,
+                 h->d_o2s,    h->d_ba,     h->d_bb,   h->d_inc_ptr, h->d_inc, h->d_H,
+                 h->d_minv,   h->d_vfree,  h->d_V,    h->d_H32,     h->d_minv32, h->d_V32};
   for (void* p : dfp)
     if (p) hipFree(p);
   if (h->stream) hipStreamDestroy(h->stream);
@@ -1222,6 +1372,20 @@ int fc3d_hip_problem_update(fc3d_hip_problem* h, const double* blocks, const dou
   if (!h) return NUMERICS_ERR_NULL_POINTER;
   int rc = use_device(h->device);
   if (rc) return rc;
+  if (blocks || q) {
+    // attached factors of W (velocity form) no longer describe the problem
+    FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
+    void* old[] = {h->d_ba, h->d_bb, h->d_inc_ptr, h->d_inc, h->d_H, h->d_minv, h->d_vfree,
+                   h->d_V, h->d_H32, h->d_minv32, h->d_V32};
+    for (void* x : old)
+      if (x) hipFree(x);
+    h->d_ba = h->d_bb = h->d_inc_ptr = h->d_inc = nullptr;
+    h->d_H = h->d_minv = h->d_vfree = nullptr;
+    h->d_V = nullptr;
+    h->d_H32 = h->d_minv32 = nullptr;
+    h->d_V32 = nullptr;
+    h->nbodies = 0;
+  }
   const size_t ns = (size_t)h->nslots;
   if (blocks && h->nnzb > 0) {
     if (!h->d_block_dst && (rc = dev_upload(&h->d_block_dst, h->block_dst))) return rc;
@@ -1291,6 +1455,139 @@ int fc3d_hip_problem_update(fc3d_hip_problem* h, const double* blocks, const dou
     if (mu) conv(h->mu, h->mu32, ns);
     FC3D_HIP_TRY(hipGetLastError());
     FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  return 0;
+}
+
+int fc3d_hip_problem_set_bodies(fc3d_hip_problem* h, int nbodies, const int* body_a,
+                                const int* body_b, const double* H, const double* Minv,
+                                int minv_diagonal, const double* vfree) {
+  if (!h || !body_a || !body_b || !H || !Minv || !vfree) return NUMERICS_ERR_NULL_POINTER;
+  if (nbodies <= 0) {
+    set_error("fc3d_hip_problem_set_bodies: nbodies must be positive");
+    return NUMERICS_ERR_INVALID_ARGUMENT;
+  }
+  const int nc = h->nc;
+  for (int c = 0; c < nc; c++)
+    if (body_a[c] < -1 || body_a[c] >= nbodies || body_b[c] < -1 || body_b[c] >= nbodies) {
+      set_error("fc3d_hip_problem_set_bodies: contact %d has a body outside [-1, %d)", c, nbodies);
+      return NUMERICS_ERR_INVALID_ARGUMENT;
+    }
+  int rc = use_device(h->device);
+  if (rc) return rc;
+  // incidence (slot, side) per body, ascending slot
+  std::vector<int> ptr((size_t)nbodies + 1, 0), inc;
+  for (int c = 0; c < nc; c++) {
+    if (body_a[c] >= 0) ptr[(size_t)body_a[c] + 1]++;
+    if (body_b[c] >= 0) ptr[(size_t)body_b[c] + 1]++;
+  }
+  for (int k = 0; k < nbodies; k++) ptr[(size_t)k + 1] += ptr[(size_t)k];
+  inc.resize((size_t)ptr[(size_t)nbodies]);
+  {
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int slot = 0; slot < h->nslots; slot++) {
+      const int c = h->slot2orig[(size_t)slot];
+      if (c < 0) continue;
+      if (body_a[c] >= 0) inc[(size_t)fill[(size_t)body_a[c]]++] = 2 * slot;
+      if (body_b[c] >= 0) inc[(size_t)fill[(size_t)body_b[c]]++] = 2 * slot + 1;
+    }
+  }
+  // the velocity update of a colour is race-free only if a body's contacts
+  // all have different colours (the pattern of W must hold every such pair)
+  {
+    std::vector<int> mark((size_t)h->ncolours, -1);
+    for (int k = 0; k < nbodies; k++)
+      for (int t = ptr[(size_t)k]; t < ptr[(size_t)k + 1]; t++) {
+        const int c = h->slot2orig[(size_t)(inc[(size_t)t] >> 1)];
+        const int col = h->colour[(size_t)c];
+        if (mark[(size_t)col] == k && !(t > ptr[(size_t)k] && (inc[(size_t)t] >> 1) ==
+                                                                   (inc[(size_t)t - 1] >> 1))) {
+          set_error("fc3d_hip_problem_set_bodies: two contacts of body %d share colour %d (W "
+                    "lacks the block between them)", k, col);
+          return NUMERICS_ERR_INVALID_ARGUMENT;
+        }
+        mark[(size_t)col] = k;
+      }
+  }
+  // the factors must reproduce the resident diagonal blocks of W
+  {
+    std::vector<double> diag((size_t)h->nslices * 9 * 64);
+    FC3D_HIP_TRY(hipMemcpy(diag.data(), h->diag, sizeof(double) * diag.size(),
+                           hipMemcpyDeviceToHost));
+    for (int c = 0; c < nc; c++) {
+      const int slot = h->orig2slot[(size_t)c], sl = slot / 64, lane = slot % 64;
+      const int kb[2] = {body_a[c], body_b[c]};
+      double w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, big = 0.0, err = 0.0;
+      for (int s1 = 0; s1 < 2; s1++)
+        for (int s2 = 0; s2 < 2; s2++) {
+          if (kb[s1] < 0 || kb[s1] != kb[s2]) continue;
+          const double* h1 = H + 36 * (size_t)c + 18 * s1;
+          const double* h2 = H + 36 * (size_t)c + 18 * s2;
+          const int k = kb[s1];
+          for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+              double v = 0.0;
+              for (int l = 0; l < 6; l++)
+                for (int m = 0; m < 6; m++) {
+                  const double mi = minv_diagonal ? (l == m ? Minv[6 * (size_t)k + l] : 0.0)
+                                                  : Minv[36 * (size_t)k + l + 6 * m];
+                  v += h1[i + 3 * l] * mi * h2[j + 3 * m];
+                }
+              w[i + 3 * j] += v;
+            }
+        }
+      for (int e = 0; e < 9; e++) {
+        const double d = diag[((size_t)sl * 9 + e) * 64 + (size_t)lane];
+        big = std::max(big, std::fabs(d));
+        err = std::max(err, std::fabs(d - w[e]));
+      }
+      if (err > 1e-10 * big + 1e-300) {
+        set_error("fc3d_hip_problem_set_bodies: H M^-1 H^T differs from W on the diagonal block "
+                  "of contact %d (%g)", c, err);
+        return NUMERICS_ERR_INVALID_ARGUMENT;
+      }
+    }
+  }
+  // per-slot bodies and H planes
+  std::vector<int> ba((size_t)h->nslots, -1), bb((size_t)h->nslots, -1);
+  std::vector<double> Hs((size_t)h->nslices * 36 * 64, 0.0);
+  for (int c = 0; c < nc; c++) {
+    const int slot = h->orig2slot[(size_t)c], sl = slot / 64, lane = slot % 64;
+    ba[(size_t)slot] = body_a[c];
+    bb[(size_t)slot] = body_b[c];
+    for (int e = 0; e < 36; e++) Hs[((size_t)sl * 36 + e) * 64 + (size_t)lane] = H[36 * (size_t)c + e];
+  }
+  // drop earlier factors, upload the new ones
+  void* old[] = {h->d_ba, h->d_bb, h->d_inc_ptr, h->d_inc, h->d_H, h->d_minv, h->d_vfree, h->d_V,
+                 h->d_H32, h->d_minv32, h->d_V32};
+  for (void* q : old)
+    if (q) hipFree(q);
+  h->d_ba = h->d_bb = h->d_inc_ptr = h->d_inc = nullptr;
+  h->d_H = h->d_minv = h->d_vfree = nullptr;
+  h->d_V = nullptr;
+  h->d_H32 = h->d_minv32 = nullptr;
+  h->d_V32 = nullptr;
+  std::vector<double> mv(Minv, Minv + (size_t)nbodies * (minv_diagonal ? 6 : 36));
+  std::vector<double> vf(vfree, vfree + 6 * (size_t)nbodies);
+  if ((rc = dev_upload(&h->d_ba, ba)) || (rc = dev_upload(&h->d_bb, bb)) ||
+      (rc = dev_upload(&h->d_inc_ptr, ptr)) || (rc = dev_upload(&h->d_inc, inc)) ||
+      (rc = dev_upload(&h->d_H, Hs)) || (rc = dev_upload(&h->d_minv, mv)) ||
+      (rc = dev_upload(&h->d_vfree, vf)) || (rc = dev_alloc(&h->d_V, 2 * (size_t)nbodies)))
+    return rc;
+  h->nbodies = nbodies;
+  h->minv_diag = minv_diagonal ? 1 : 0;
+  // algorithmic bytes per contact of one velocity-form colour pass: H, W_cc,
+  // mu, r (read, write), PLI step (read, write), two body ids, two bodies'
+  // velocities (read, write) and M^-1
+  const double mb = minv_diagonal ? 6.0 : 36.0;
+  const double per64 = 8.0 * (36 + 9 + 1 + 2 + 4 + 4 + 2 * 2 * 8 + 2 * mb) + 8.0;
+  const double per32 = 4.0 * (36 + 9 + 1 + 2 + 4 + 4 + 2 * 2 * 8 + 2 * mb) + 8.0;
+  h->vel_bytes64.assign((size_t)h->ncolours, 0.0);
+  h->vel_bytes32.assign((size_t)h->ncolours, 0.0);
+  for (int c = 0; c < nc; c++) {
+    const int col = h->colour[(size_t)c];
+    h->vel_bytes64[(size_t)col] += per64;
+    h->vel_bytes32[(size_t)col] += per32;
   }
   return 0;
 }
@@ -1365,6 +1662,15 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
               p->accel_depth, AA_MAX, p->accel_every, p->accel_reg);
     return NUMERICS_ERR_INVALID_ARGUMENT;
   }
+  if (p->velocity_form && !h->d_H) {
+    set_error("fc3d_hip_solve: velocity_form needs the factors of W "
+              "(fc3d_hip_problem_set_bodies)");
+    return NUMERICS_ERR_INVALID_OPTION;
+  }
+  if (p->velocity_form && p->warm_start >= 2) {
+    set_error("fc3d_hip_solve: velocity_form with a learned start is not available");
+    return NUMERICS_ERR_NOT_IMPLEMENTED;
+  }
   if (p->warm_start >= 2 && !h->model) {
     set_error("fc3d_hip_solve: warm_start %d needs a model (fc3d_hip_problem_set_model)",
               p->warm_start);
@@ -1386,13 +1692,16 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
   const int mixed = p->precision == 1;
   if (mixed && (rc = ensure_fp32(h))) return rc;
 
-  h->accel_on = 0;  // the autotuner times plain sweeps
+  h->accel_on = 0;  // the autotuner times plain matrix-form sweeps
+  h->vel_on = 0;
   int sched[2] = {p->schedule, p->schedule};
   if (p->schedule < 0 || p->schedule > FC3D_NUM_SCHEDULES) {
     set_error("fc3d_hip_solve: schedule %d out of range [0, %d]", p->schedule, FC3D_NUM_SCHEDULES);
     return NUMERICS_ERR_INVALID_ARGUMENT;
   }
-  if (p->schedule == 0) {
+  if (p->schedule == 0 && p->velocity_form) {
+    sched[0] = sched[1] = 1;  // the velocity form has one kernel of its own
+  } else if (p->schedule == 0) {
     for (int ph = 0; ph <= mixed; ph++) {
       if (!h->tuned_schedule[ph]) {
         if (ph == 1)
@@ -1413,6 +1722,10 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
   if (p->accel_depth > 0) {
     if ((rc = ensure_accel(h, mixed))) return rc;
     h->accel_on = 1;
+  }
+  if (p->velocity_form) {
+    if (mixed && (rc = ensure_vel32(h))) return rc;
+    h->vel_on = 1;
   }
 
   SolveState s0;
@@ -1466,6 +1779,7 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
       if (phase == 0 && mixed && (rc = accel_stats(h, s))) return rc;
       if ((rc = reset_accel(h))) return rc;
     }
+    if (h->vel_on) launch_vinit(h, phase, nullptr, nullptr);  // v from the phase's reaction
     // every replay advances the sweep counter or stops; the bound only
     // guards against a launch that silently did not run
     const long max_replays = (long)p->max_iter / (p->instrument ? 1 : chunk) + 4;

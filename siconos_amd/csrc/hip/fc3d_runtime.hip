@@ -67,6 +67,7 @@ void fc3d_hip_params_default(fc3d_hip_params* p) {
   p->accel_depth = 0;
   p->accel_every = 2;
   p->accel_reg = 1e-10;
+  p->velocity_form = 0;
 }
 
 }  // extern "C"

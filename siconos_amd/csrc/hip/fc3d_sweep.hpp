@@ -148,6 +148,69 @@ __device__ __forceinline__ int xcd_block() {
   return x * q + (x < rem ? x : rem) + (b >> 3);
 }
 
+// Inputs of one slot's local solve: diagonal block, mu, the old reaction and
+// the persistent PLI step; q_loc starts at the slot's q.
+template <typename T>
+struct SlotInputs {
+  LocalProblem<T> lp;
+  typename vec4<T>::type ro;
+  T pli;
+  T ql[3];
+};
+
+template <typename T>
+__device__ __forceinline__ void load_slot(const SweepPtrs<T>& P, int s, int lane, SlotInputs<T>* in) {
+  const int slot = s * 64 + lane;
+#pragma unroll
+  for (int e = 0; e < 9; e++) in->lp.W[e] = P.diag[(size_t)(s * 9 + e) * 64 + lane];
+  in->lp.mu = P.mu[slot];
+  in->ro = P.r[slot];
+  in->pli = P.pli[slot];
+#pragma unroll
+  for (int e = 0; e < 3; e++) in->ql[e] = P.q[(size_t)(s * 3 + e) * 64 + lane];
+}
+
+// Local solve, relaxation, light-error term, optional filter and write-back
+// for a slot whose local q (q + sum_j W_ij r_j) is in in->ql.
+template <typename T>
+__device__ __forceinline__ void finish_slot(const SweepPtrs<T>& P, const DevParams* __restrict__ prm,
+                                            int s, int lane, T tol, SlotInputs<T>& in,
+                                            double* light, double* nr2,
+                                            typename vec4<T>::type* out_r = nullptr) {
+  const int slot = s * 64 + lane;
+  LocalProblem<T>& lp = in.lp;
+  const typename vec4<T>::type ro = in.ro;
+  T pli = in.pli;
+  lp.q[0] = in.ql[0]; lp.q[1] = in.ql[1]; lp.q[2] = in.ql[2];
+  const T rold[3] = {ro.x, ro.y, ro.z};
+  T lr[3] = {ro.x, ro.y, ro.z};
+  T rho[3];
+  compute_rho<T>(prm->rho_strategy, T(prm->rho_constant), lp.W, rho);
+  T res = T(0);
+  local_solve<T>(lp, lr, prm->lp, tol, &pli, rho, &res);
+  if (prm->relaxation) {
+    const T om = T(prm->omega);
+#pragma unroll
+    for (int e = 0; e < 3; e++) lr[e] = om * lr[e] + (T(1) - om) * rold[e];
+  }
+  const T d0 = rold[0] - lr[0], d1 = rold[1] - lr[1], d2 = rold[2] - lr[2];
+  *light = double(d0 * d0 + d1 * d1 + d2 * d2);
+  const bool accept = !(prm->filter && (isnan(res) || isinf(res) || res > T(1)));
+  typename vec4<T>::type out = ro;
+  if (accept) { out.x = lr[0]; out.y = lr[1]; out.z = lr[2]; }
+  P.r[slot] = out;
+  P.pli[slot] = pli;
+  if (out_r) *out_r = out;
+  if (P.f) {
+    typename vec4<T>::type d = out;
+    d.x -= ro.x; d.y -= ro.y; d.z -= ro.z; d.w = T(0);
+    P.f[slot] = d;
+  }
+  *nr2 = sizeof(T) == 8 ? double(out.x * out.x + out.y * out.y + out.z * out.z)
+                        : double(out.x) * double(out.x) + double(out.y) * double(out.y) +
+                              double(out.z) * double(out.z);
+}
+
 // The per-contact step of a sweep (fc3d_nsgs.c:737-822) for the lane's slot of
 // slice s: local problem (fc3d_nsgs_update), local solve, relaxation, light
 // error term, optional filter, write-back.  Returns |r_old - r_new|^2 and the
@@ -157,47 +220,49 @@ __device__ __forceinline__ void sweep_slot(const SweepPtrs<T>& P, const DevParam
                                            int s, int lane, T tol, double* light, double* nr2) {
   *light = 0.0;
   *nr2 = 0.0;
-  const int slot = s * 64 + lane;
-  const int len = P.len[slot];
+  const int len = P.len[s * 64 + lane];
   if (len >= 0) {
     // local-solve inputs first: their loads overlap the gather
-    LocalProblem<T> lp;
+    SlotInputs<T> in;
+    load_slot<T>(P, s, lane, &in);
+    gather_row<T, U, PF>(P, s, lane, len, in.ql);
+    finish_slot<T>(P, prm, s, lane, tol, in, light, nr2);
+  }
+}
+
+// Partial gather for the four-wave sweep: planes k = first, first + step, ...
+// of the slice, blocks k < len only, summed into y (order of the planes taken;
+// not the reference row order, so used only where bit-exactness is not kept).
+template <typename T, int U>
+__device__ __forceinline__ void gather_planes(const SweepPtrs<T>& P, int s, int lane, int len,
+                                              int first, int step, T* y) {
+  const int koff = __builtin_amdgcn_readfirstlane(P.koff[s]);
+  const int kmax = __builtin_amdgcn_readfirstlane(P.kmax[s]);
+  const int* __restrict__ colp = P.col + (size_t)koff * 64 + lane;
+  const T* __restrict__ valp = P.vals + (size_t)koff * 9 * 64 + lane;
+  for (int k0 = first; k0 < kmax; k0 += U * step) {
+    int j[U];
+    T w[U][9];
+    typename vec4<T>::type rj[U];
 #pragma unroll
-    for (int e = 0; e < 9; e++) lp.W[e] = P.diag[(size_t)(s * 9 + e) * 64 + lane];
-    lp.mu = P.mu[slot];
-    const typename vec4<T>::type ro = P.r[slot];
-    T pli = P.pli[slot];
-    T ql[3];
+    for (int u = 0; u < U; u++) {
+      const int k = k0 + u * step;
+      if (k < kmax) {
+        j[u] = colp[(size_t)k * 64];
 #pragma unroll
-    for (int e = 0; e < 3; e++) ql[e] = P.q[(size_t)(s * 3 + e) * 64 + lane];
-    gather_row<T, U, PF>(P, s, lane, len, ql);
-    lp.q[0] = ql[0]; lp.q[1] = ql[1]; lp.q[2] = ql[2];
-    const T rold[3] = {ro.x, ro.y, ro.z};
-    T lr[3] = {ro.x, ro.y, ro.z};
-    T rho[3];
-    compute_rho<T>(prm->rho_strategy, T(prm->rho_constant), lp.W, rho);
-    T res = T(0);
-    local_solve<T>(lp, lr, prm->lp, tol, &pli, rho, &res);
-    if (prm->relaxation) {
-      const T om = T(prm->omega);
-#pragma unroll
-      for (int e = 0; e < 3; e++) lr[e] = om * lr[e] + (T(1) - om) * rold[e];
+        for (int e = 0; e < 9; e++) w[u][e] = valp[((size_t)k * 9 + e) * 64];
+      }
     }
-    const T d0 = rold[0] - lr[0], d1 = rold[1] - lr[1], d2 = rold[2] - lr[2];
-    *light = double(d0 * d0 + d1 * d1 + d2 * d2);
-    const bool accept = !(prm->filter && (isnan(res) || isinf(res) || res > T(1)));
-    typename vec4<T>::type out = ro;
-    if (accept) { out.x = lr[0]; out.y = lr[1]; out.z = lr[2]; }
-    P.r[slot] = out;
-    P.pli[slot] = pli;
-    if (P.f) {
-      typename vec4<T>::type d = out;
-      d.x -= ro.x; d.y -= ro.y; d.z -= ro.z; d.w = T(0);
-      P.f[slot] = d;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (k0 + u * step < kmax) rj[u] = P.r[j[u]];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (k0 + u * step < len) {
+        const T v[3] = {rj[u].x, rj[u].y, rj[u].z};
+        mvp3x3<T>(w[u], v, y);
+      }
     }
-    *nr2 = sizeof(T) == 8 ? double(out.x * out.x + out.y * out.y + out.z * out.z)
-                          : double(out.x) * double(out.x) + double(out.y) * double(out.y) +
-                                double(out.z) * double(out.z);
   }
 }
 

@@ -1,0 +1,157 @@
+// fc3d_velocity.hpp — the velocity form of the coloured sweep (no reference
+// counterpart; enabled by fc3d_hip_params.velocity_form once the factors of
+// W = H M^-1 H^T are attached with fc3d_hip_problem_set_bodies).
+//
+// With body velocities v = v_free + M^-1 H^T r kept up to date, the local
+// velocity of contact c is u_c = H_{c,a} v_a + H_{c,b} v_b, so
+//     q_loc = q_c + sum_{j != c} W_cj r_j = u_c - W_cc r_c,
+// the same local problem as the matrix form in exact arithmetic.  After the
+// local solve, v_a += M_a^-1 H_{c,a}^T (r_new - r_old) and likewise for b.
+// Contacts of one colour share no body, so a colour's updates never collide.
+// Per contact this reads the 3x6 blocks of H and two body velocities instead
+// of the ~k neighbour blocks of W and k neighbour reactions.
+#pragma once
+#include "fc3d_accel.hpp"
+
+namespace fc3d {
+
+template <typename T>
+struct BodyPtrs {
+  const int* ba;     // per slot: body of side a, -1 for none or padding
+  const int* bb;     // per slot: body of side b
+  const T* H;        // [slice][36][64]: side a entries 0-17, side b 18-35 (3x6 column-major)
+  const T* minv;     // per body: 6 diagonal entries, or 36 (6x6 column-major)
+  int minv_diag;
+  typename vec4<T>::type* V;  // per body: (v0, v1, v2), (v3, v4, v5)
+  const double* vfree;        // per body: 6
+  const int* inc_ptr;         // per body: range in inc
+  const int* inc;             // 2 slot + side, ascending slot
+};
+
+template <typename T>
+__device__ __forceinline__ void minv_apply(const BodyPtrs<T>& B, int k, const T* g, T* dv) {
+  if (B.minv_diag) {
+    const T* m = B.minv + 6 * (size_t)k;
+#pragma unroll
+    for (int l = 0; l < 6; l++) dv[l] = m[l] * g[l];
+  } else {
+    const T* m = B.minv + 36 * (size_t)k;
+#pragma unroll
+    for (int l = 0; l < 6; l++) {
+      T s = T(0);
+#pragma unroll
+      for (int t = 0; t < 6; t++) s += m[l + 6 * t] * g[t];
+      dv[l] = s;
+    }
+  }
+}
+
+// v_k = v_free,k + M_k^-1 sum_{c at k} H_{c,k}^T r_c for every body (in a fixed
+// order of its contacts).  With `as`, only after an Anderson step changed r.
+template <typename T>
+__global__ void k_vinit(BodyPtrs<T> B, const typename vec4<T>::type* __restrict__ r, int nbodies,
+                        const SolveState* st, const AccelState* as, int want_phase) {
+  if (st && (st->stop || st->phase != want_phase)) return;
+  if (as && !as->apply) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nbodies) return;
+  T g[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  for (int t = B.inc_ptr[k]; t < B.inc_ptr[k + 1]; t++) {
+    const int e = B.inc[t], slot = e >> 1, side = e & 1;
+    const int s = slot >> 6, lane = slot & 63;
+    const typename vec4<T>::type rc = r[slot];
+    const T* h = B.H + (size_t)s * 36 * 64 + (size_t)(18 * side) * 64 + lane;
+#pragma unroll
+    for (int l = 0; l < 6; l++)
+      g[l] += h[(size_t)(3 * l) * 64] * rc.x + h[(size_t)(3 * l + 1) * 64] * rc.y +
+              h[(size_t)(3 * l + 2) * 64] * rc.z;
+  }
+  T dv[6];
+  minv_apply<T>(B, k, g, dv);
+  typename vec4<T>::type v0, v1;
+  v0.x = T(B.vfree[6 * (size_t)k]) + dv[0];
+  v0.y = T(B.vfree[6 * (size_t)k + 1]) + dv[1];
+  v0.z = T(B.vfree[6 * (size_t)k + 2]) + dv[2];
+  v0.w = T(0);
+  v1.x = T(B.vfree[6 * (size_t)k + 3]) + dv[3];
+  v1.y = T(B.vfree[6 * (size_t)k + 4]) + dv[4];
+  v1.z = T(B.vfree[6 * (size_t)k + 5]) + dv[5];
+  v1.w = T(0);
+  B.V[2 * (size_t)k] = v0;
+  B.V[2 * (size_t)k + 1] = v1;
+}
+
+// One colour of one sweep in the velocity form.
+template <typename T>
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void k_sweep_vel(SweepPtrs<T> P,
+                                                                          BodyPtrs<T> B,
+                                                      const DevParams* __restrict__ prm,
+                                                      const SolveState* __restrict__ st,
+                                                      int s_begin, int s_end, int want_phase) {
+  if (st->stop || st->phase != want_phase) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s = __builtin_amdgcn_readfirstlane(s_begin + xcd_block() * 4 + wave);
+  if (s >= s_end) return;
+  const int slot = s * 64 + lane;
+  double light = 0.0, nr2 = 0.0;
+  if (P.len[slot] >= 0) {
+    const T tol = T(want_phase ? prm->fp32_local_tol : st->local_tol);
+    SlotInputs<T> in;
+    load_slot<T>(P, s, lane, &in);
+    const int kb[2] = {B.ba[slot], B.bb[slot]};
+    T h[36];
+#pragma unroll
+    for (int e = 0; e < 36; e++) h[e] = B.H[((size_t)s * 36 + e) * 64 + lane];
+    typename vec4<T>::type va[2][2];
+#pragma unroll
+    for (int side = 0; side < 2; side++)
+      if (kb[side] >= 0) {
+        va[side][0] = B.V[2 * (size_t)kb[side]];
+        va[side][1] = B.V[2 * (size_t)kb[side] + 1];
+      }
+    // u = H_a v_a + H_b v_b, then q_loc = u - W_cc r_old
+    T u[3] = {T(0), T(0), T(0)};
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+      if (kb[side] < 0) continue;
+      const T v[6] = {va[side][0].x, va[side][0].y, va[side][0].z,
+                      va[side][1].x, va[side][1].y, va[side][1].z};
+#pragma unroll
+      for (int l = 0; l < 6; l++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) u[i] += h[18 * side + i + 3 * l] * v[l];
+    }
+    const T rold[3] = {in.ro.x, in.ro.y, in.ro.z};
+    T wr[3] = {T(0), T(0), T(0)};
+    mvp3x3<T>(in.lp.W, rold, wr);
+#pragma unroll
+    for (int i = 0; i < 3; i++) in.ql[i] = u[i] - wr[i];
+    typename vec4<T>::type out;
+    finish_slot<T>(P, prm, s, lane, tol, in, &light, &nr2, &out);
+    const T dr[3] = {out.x - rold[0], out.y - rold[1], out.z - rold[2]};
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+      if (kb[side] < 0) continue;
+      T g[6], dv[6];
+#pragma unroll
+      for (int l = 0; l < 6; l++)
+        g[l] = h[18 * side + 3 * l] * dr[0] + h[18 * side + 3 * l + 1] * dr[1] +
+               h[18 * side + 3 * l + 2] * dr[2];
+      minv_apply<T>(B, kb[side], g, dv);
+      // the other side may be the same body: read back what this thread wrote
+      typename vec4<T>::type v0 = B.V[2 * (size_t)kb[side]], v1 = B.V[2 * (size_t)kb[side] + 1];
+      v0.x += dv[0]; v0.y += dv[1]; v0.z += dv[2];
+      v1.x += dv[3]; v1.y += dv[4]; v1.z += dv[5];
+      B.V[2 * (size_t)kb[side]] = v0;
+      B.V[2 * (size_t)kb[side] + 1] = v1;
+    }
+  }
+  light = wave_sum(light);
+  nr2 = wave_sum(nr2);
+  if (lane == 0) {
+    P.part[2 * (size_t)s] = light;
+    P.part[2 * (size_t)s + 1] = nr2;
+  }
+}
+
+}  // namespace fc3d

@@ -362,6 +362,17 @@ class HipProblem:
         self._check(self._lib.fc3d_hip_problem_update(
             self._h, *[None if v is None else dptr(v) for v in a]), "update")
 
+    def set_bodies(self, nbodies, body_a, body_b, H, minv, vfree, minv_diagonal=True):
+        """Factors of W for velocity-form sweeps (fc3d_hip_problem_set_bodies)."""
+        a = np.ascontiguousarray(body_a, dtype=np.int32)
+        b = np.ascontiguousarray(body_b, dtype=np.int32)
+        H = np.ascontiguousarray(H, dtype=np.float64)
+        minv = np.ascontiguousarray(minv, dtype=np.float64)
+        vfree = np.ascontiguousarray(vfree, dtype=np.float64)
+        self._check(self._lib.fc3d_hip_problem_set_bodies(
+            self._h, nbodies, iptr(a), iptr(b), dptr(H), dptr(minv), int(minv_diagonal),
+            dptr(vfree)), "set_bodies")
+
     def set_reaction(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64)
         self._check(self._lib.fc3d_hip_set_reaction(self._h, dptr(r)), "set_reaction")

@@ -203,9 +203,9 @@ def pile50k_reference(oracle):
     return g, H, ro, it
 
 
-@pytest.mark.parametrize("schedule", range(1, 10))
+@pytest.mark.parametrize("schedule", range(1, 11))
 def test_every_sweep_schedule_is_bit_exact(pile50k_reference, schedule):
-    """Schedules 1-6, 8, 9 launch one kernel per colour with different unrolling,
+    """Schedules 1-6, 8-10 launch one kernel per colour with different unrolling,
     prefetch and occupancy; schedule 7 runs all colours in one launch, each
     slice waiting for its lower-colour neighbours.  All of them must give the
     oracle's colour-order result bit for bit."""
@@ -225,3 +225,14 @@ def test_single_launch_sweep_with_fp32_anderson_and_timing(pile50k_reference):
     assert s.info == 0 and s.iterations == it
     assert s.sweep_kernel_launches == it  # one launch per sweep
     np.testing.assert_array_equal(H.reaction(), ro)
+
+
+@pytest.mark.parametrize("schedule", [1, 8, 10])
+def test_fp32_sweep_schedules_reach_the_fp64_tolerance(pile50k_reference, schedule):
+    """Schedule 10 gathers with four waves per slice in FP32 (another summation
+    order); every FP32 schedule must still end at the FP64 tolerance."""
+    g, H, ro, it = pile50k_reference
+    s = H.solve(N.hip_params(tol=1e-6, max_iter=3000, schedule=schedule, precision=1,
+                             fp32_tol=1e-6, warm_start=1))
+    assert s.info == 0 and s.residual <= 1e-6 and s.fp32_sweeps > 0
+    assert abs(H.compute_error() - s.residual) <= 1e-12 + 1e-9 * s.residual

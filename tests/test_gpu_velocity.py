@@ -1,0 +1,74 @@
+"""Velocity-form sweeps (fc3d_hip_params.velocity_form with the factors of W
+attached by fc3d_hip_problem_set_bodies).
+
+In exact arithmetic the velocity form is the same Gauss-Seidel iteration as
+the matrix form, because u_c - W_cc r_c = q_c + sum_{j != c} W_cj r_j when
+v = v_free + M^-1 H^T r. The tests compare the two forms' iterates, which
+differ only by rounding, and check that solves end at the FP64 tolerance
+judged with W.
+"""
+import numpy as np
+import pytest
+
+from siconos_amd import numerics as N
+
+pytestmark = pytest.mark.gpu
+
+
+def _with_bodies(g):
+    H = N.HipProblem(g.W, g.q, g.mu)
+    H.set_bodies(g.nbodies, g.body_a, g.body_b, g.H, g.minv, g.vfree)
+    return H
+
+
+@pytest.mark.parametrize("K", [1, 3, 10, 30])
+def test_iterates_match_the_matrix_form(oracle, K):
+    g = N.generate_random_pairs(300, 1000, 0.2, 0.5, 1)
+    H = _with_bodies(g)
+    H.solve(N.hip_params(tol=1e-300, max_iter=K, warm_start=1, velocity_form=1))
+    rv = H.reaction()
+    H.solve(N.hip_params(tol=1e-300, max_iter=K, warm_start=1, velocity_form=0))
+    rm = H.reaction()
+    info, ro, uo, it, res = oracle.nsgs(g.W, g.q, g.mu, tol=1e-300, max_iter=K, order=H.order())
+    np.testing.assert_array_equal(rm, ro)  # the matrix form is the oracle, bit for bit
+    assert np.linalg.norm(rv - ro) <= 1e-10 * np.linalg.norm(ro)
+
+
+@pytest.mark.parametrize("kind", ["pairs", "pile"])
+def test_solves_end_at_the_fp64_tolerance(lib, kind):
+    if kind == "pairs":
+        g = N.generate_random_pairs(20000, 50000, 0.2, 0.5, 2)
+    else:
+        g = N.generate_granular_pile(20000, 50000, 0.1, 0.9, 5)
+    H = _with_bodies(g)
+    m = H.solve(N.hip_params(tol=1e-6, max_iter=3000, warm_start=1))
+    v = H.solve(N.hip_params(tol=1e-6, max_iter=3000, warm_start=1, velocity_form=1))
+    assert v.info == 0 and v.residual <= 1e-6
+    assert abs(v.iterations - m.iterations) <= max(3, 0.02 * m.iterations)
+    assert abs(H.compute_error() - v.residual) <= 1e-12 + 1e-9 * v.residual
+    f = H.solve(N.hip_params(tol=1e-6, max_iter=3000, warm_start=1, velocity_form=1, precision=1,
+                             fp32_tol=1e-6, accel_depth=3))
+    assert f.info == 0 and f.residual <= 1e-6 and f.fp32_sweeps > 0
+    print(f"\n[velocity] {kind}: matrix {m.iterations} sweeps, velocity {v.iterations}, "
+          f"fp32+anderson velocity {f.iterations} ({f.accel_steps} steps)")
+
+
+def test_factors_are_checked_and_dropped_by_updates(lib):
+    g = N.generate_granular_pile(4000, 10000, 0.1, 0.9, 3)
+    H = N.HipProblem(g.W, g.q, g.mu)
+    with pytest.raises(RuntimeError, match="-3"):
+        H.solve(N.hip_params(velocity_form=1))  # no factors yet
+    bad = g.H.copy()
+    bad[5] *= 1.5
+    with pytest.raises(RuntimeError, match="-2"):
+        H.set_bodies(g.nbodies, g.body_a, g.body_b, bad, g.minv, g.vfree)
+    with pytest.raises(RuntimeError, match="-2"):
+        H.set_bodies(g.nbodies, g.body_a, g.body_b + g.nbodies, g.H, g.minv, g.vfree)
+    H.set_bodies(g.nbodies, g.body_a, g.body_b, g.H, g.minv, g.vfree)
+    with pytest.raises(RuntimeError, match="-5"):
+        H.solve(N.hip_params(velocity_form=1, warm_start=2))
+    s = H.solve(N.hip_params(tol=1e-6, max_iter=3000, warm_start=1, velocity_form=1))
+    assert s.info == 0
+    H.update(q=g.q)  # a new q: the factors (and v_free) may no longer match
+    with pytest.raises(RuntimeError, match="-3"):
+        H.solve(N.hip_params(velocity_form=1))
