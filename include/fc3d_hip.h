@@ -95,6 +95,8 @@ typedef struct fc3d_hip_stats {
   double start_residual_learned;
   int accel_steps;            /* Anderson steps taken                            */
   int accel_restarts;         /* history restarts                                */
+  int accel_rejects;          /* steps undone by the safeguard (each costs the   *
+                               * sweep that judged it)                           */
 } fc3d_hip_stats;
 
 FC3D_API int fc3d_hip_device_count(void);

@@ -85,7 +85,8 @@ class HipStats(C.Structure):
                 ("sweep_kernel_ms", C.c_double), ("sweep_kernel_launches", C.c_int64),
                 ("sweep_kernel_bytes", C.c_double), ("schedule", C.c_int),
                 ("start_residual_fallback", C.c_double), ("start_residual_learned", C.c_double),
-                ("accel_steps", C.c_int), ("accel_restarts", C.c_int)]
+                ("accel_steps", C.c_int), ("accel_restarts", C.c_int),
+                ("accel_rejects", C.c_int)]
 
 
 class HipBatchStats(C.Structure):

@@ -827,9 +827,14 @@ static void enqueue_sweep(fc3d_hip_problem* h, int phase, int schedule, hipEvent
   }
   if (h->accel_on) {
     const dim3 g((unsigned)h->aa_nblocks), b(256);
+    const int nv = 2 * h->nbodies;  // velocity form: two vec4 per body
+    const dim3 gv((unsigned)std::max(1, std::min((nv + 255) / 256, 1024)));
     if (phase) {
       hipLaunchKernelGGL(k_aa_update<float>, g, b, 0, h->stream, h->r32, h->aa32, h->st, h->ast,
                          h->prm, h->nslots, 1);
+      if (h->vel_on)
+        hipLaunchKernelGGL(k_aa_update_v<float>, gv, b, 0, h->stream, h->d_V32, h->aa32, h->st,
+                           h->ast, nv, 1);
       hipLaunchKernelGGL(k_aa_finalize, dim3(1), dim3(1024), 0, h->stream, h->aa32.part,
                          h->aa_nblocks, h->st, h->ast, h->prm, 1);
       hipLaunchKernelGGL(k_aa_apply<float>, g, b, 0, h->stream, h->r32, h->aa32, h->st, h->ast,
@@ -837,13 +842,24 @@ static void enqueue_sweep(fc3d_hip_problem* h, int phase, int schedule, hipEvent
     } else {
       hipLaunchKernelGGL(k_aa_update<double>, g, b, 0, h->stream, h->r, h->aa64, h->st, h->ast,
                          h->prm, h->nslots, 0);
+      if (h->vel_on)
+        hipLaunchKernelGGL(k_aa_update_v<double>, gv, b, 0, h->stream, h->d_V, h->aa64, h->st,
+                           h->ast, nv, 0);
       hipLaunchKernelGGL(k_aa_finalize, dim3(1), dim3(1024), 0, h->stream, h->aa64.part,
                          h->aa_nblocks, h->st, h->ast, h->prm, 0);
       hipLaunchKernelGGL(k_aa_apply<double>, g, b, 0, h->stream, h->r, h->aa64, h->st, h->ast,
                          h->prm, h->nslots, 0);
     }
-    // an Anderson step moved r: the body velocities follow
-    if (h->vel_on) launch_vinit(h, phase, h->st, h->ast);
+    // an Anderson step moved r: the body velocities take the same step
+    if (h->vel_on) {
+      if (phase) {
+        hipLaunchKernelGGL(k_aa_apply_v<float>, gv, b, 0, h->stream, h->d_V32, h->aa32, h->st,
+                           h->ast, h->prm, nv, 1);
+      } else {
+        hipLaunchKernelGGL(k_aa_apply_v<double>, gv, b, 0, h->stream, h->d_V, h->aa64, h->st,
+                           h->ast, h->prm, nv, 0);
+      }
+    }
   }
 }
 
@@ -864,8 +880,19 @@ static int alloc_accel(fc3d_hip_problem* h, AccelBufs<T>* A) {
   return 0;
 }
 
+// Body-velocity history of one phase (velocity form), sized for the attached
+// bodies; dropped with the factors.
+template <typename T>
+static int alloc_accel_bodies(fc3d_hip_problem* h, AccelBufs<T>* A) {
+  if (A->vprev) return 0;
+  const size_t nv = 2 * (size_t)h->nbodies;
+  int rc;
+  if ((rc = dev_alloc(&A->vprev, nv)) || (rc = dev_alloc(&A->dV, nv * AA_MAX))) return rc;
+  return 0;
+}
+
 static int ensure_accel(fc3d_hip_problem* h, int mixed) {
-  h->aa_nblocks = (h->nslots + 255) / 256;
+  h->aa_nblocks = std::min((h->nslots + 255) / 256, 1024);  // grid-stride update
   int rc;
   if (!h->ast && (rc = dev_alloc(&h->ast, 1))) return rc;
   if ((rc = alloc_accel(h, &h->aa64))) return rc;
@@ -891,12 +918,42 @@ static int ensure_vel32(fc3d_hip_problem* h) {
   return 0;
 }
 
+static void free_accel_bodies(fc3d_hip_problem* h) {
+  void* ptrs[] = {h->aa64.vprev, h->aa64.dV, h->aa32.vprev, h->aa32.dV};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  h->aa64.vprev = h->aa64.dV = nullptr;
+  h->aa32.vprev = h->aa32.dV = nullptr;
+}
+
+// Frees the attached factors of W and everything sized by the bodies.
+static void drop_bodies(fc3d_hip_problem* h) {
+  void* old[] = {h->d_ba, h->d_bb, h->d_inc_ptr, h->d_inc, h->d_H, h->d_minv, h->d_vfree, h->d_V,
+                 h->d_H32, h->d_minv32, h->d_V32};
+  for (void* q : old)
+    if (q) hipFree(q);
+  h->d_ba = h->d_bb = h->d_inc_ptr = h->d_inc = nullptr;
+  h->d_H = h->d_minv = h->d_vfree = nullptr;
+  h->d_V = nullptr;
+  h->d_H32 = h->d_minv32 = nullptr;
+  h->d_V32 = nullptr;
+  h->nbodies = 0;
+  free_accel_bodies(h);
+  // captured velocity-form sweeps hold the freed pointers
+  for (int ph = 0; ph < 2; ph++)
+    if (h->graph[ph] && h->graph_vel[ph]) {
+      hipGraphExecDestroy(h->graph[ph]);
+      h->graph[ph] = nullptr;
+    }
+}
+
 static void free_accel(fc3d_hip_problem* h) {
   void* ptrs[] = {h->aa64.f,  h->aa64.fprev, h->aa64.gprev, h->aa64.dF, h->aa64.dG,
                   h->aa64.part, h->aa32.f,   h->aa32.fprev, h->aa32.gprev, h->aa32.dF,
                   h->aa32.dG, h->aa32.part, h->ast};
   for (void* p : ptrs)
     if (p) hipFree(p);
+  free_accel_bodies(h);
 }
 
 static int accel_stats(fc3d_hip_problem* h, fc3d_hip_stats* s) {
@@ -905,6 +962,7 @@ static int accel_stats(fc3d_hip_problem* h, fc3d_hip_stats* s) {
   FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
   s->accel_steps += a.steps;
   s->accel_restarts += a.resets;
+  s->accel_rejects += a.rejects;
   return 0;
 }
 
@@ -1375,16 +1433,7 @@ int fc3d_hip_problem_update(fc3d_hip_problem* h, const double* blocks, const dou
   if (blocks || q) {
     // attached factors of W (velocity form) no longer describe the problem
     FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
-    void* old[] = {h->d_ba, h->d_bb, h->d_inc_ptr, h->d_inc, h->d_H, h->d_minv, h->d_vfree,
-                   h->d_V, h->d_H32, h->d_minv32, h->d_V32};
-    for (void* x : old)
-      if (x) hipFree(x);
-    h->d_ba = h->d_bb = h->d_inc_ptr = h->d_inc = nullptr;
-    h->d_H = h->d_minv = h->d_vfree = nullptr;
-    h->d_V = nullptr;
-    h->d_H32 = h->d_minv32 = nullptr;
-    h->d_V32 = nullptr;
-    h->nbodies = 0;
+    drop_bodies(h);
   }
   const size_t ns = (size_t)h->nslots;
   if (blocks && h->nnzb > 0) {
@@ -1558,15 +1607,7 @@ int fc3d_hip_problem_set_bodies(fc3d_hip_problem* h, int nbodies, const int* bod
     for (int e = 0; e < 36; e++) Hs[((size_t)sl * 36 + e) * 64 + (size_t)lane] = H[36 * (size_t)c + e];
   }
   // drop earlier factors, upload the new ones
-  void* old[] = {h->d_ba, h->d_bb, h->d_inc_ptr, h->d_inc, h->d_H, h->d_minv, h->d_vfree, h->d_V,
-                 h->d_H32, h->d_minv32, h->d_V32};
-  for (void* q : old)
-    if (q) hipFree(q);
-  h->d_ba = h->d_bb = h->d_inc_ptr = h->d_inc = nullptr;
-  h->d_H = h->d_minv = h->d_vfree = nullptr;
-  h->d_V = nullptr;
-  h->d_H32 = h->d_minv32 = nullptr;
-  h->d_V32 = nullptr;
+  drop_bodies(h);
   std::vector<double> mv(Minv, Minv + (size_t)nbodies * (minv_diagonal ? 6 : 36));
   std::vector<double> vf(vfree, vfree + 6 * (size_t)nbodies);
   if ((rc = dev_upload(&h->d_ba, ba)) || (rc = dev_upload(&h->d_bb, bb)) ||
@@ -1725,6 +1766,9 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
   }
   if (p->velocity_form) {
     if (mixed && (rc = ensure_vel32(h))) return rc;
+    if (p->accel_depth > 0 && ((rc = alloc_accel_bodies(h, &h->aa64)) ||
+                               (mixed && (rc = alloc_accel_bodies(h, &h->aa32)))))
+      return rc;
     h->vel_on = 1;
   }
 

@@ -23,24 +23,41 @@ def _local_params(o, **kw):
                         ls_max_iter=int(o.iparam[12]), **kw)
 
 
-def anderson_mirror(sweep, K, m, every, reg, n3):
+def anderson_mirror(sweep, K, m, every, reg, n3, counts=None):
     """x_{k+1} = G(x_k) - dG gamma on every `every`-th sweep, history of m
-    differences restarted when |f| grows; the last sweep takes no step (the
-    solve stops there)."""
+    differences restarted when |f| grows; a step whose next sweep has a larger
+    |f| than the sweep it extrapolated is undone (back to G(x_k)), the history
+    restarts and steps pause for a doubling back-off; the last sweep takes no
+    step (the solve stops there)."""
     r = np.zeros(n3)
     F, G = [], []
     fprev = gprev = None
     nf_prev = np.inf
-    sweeps = 0
+    sweeps = steps = rejects = 0
+    pending, nf_ref, backoff, wait_until = False, 0.0, 0, 0
     for k in range(1, K + 1):
         gx = sweep(r)
         f = gx - r
         nf = np.sqrt(np.sum(f * f))
         if k == K:
+            if counts is not None:
+                counts.update(steps=steps, rejects=rejects)
             return gx
+        sweeps += 1
+        if pending and nf > nf_ref:
+            pending = False
+            rejects += 1
+            r = gprev  # G(x_k) of the rejected step
+            F, G = [], []
+            fprev = gprev = None
+            backoff = min(max(2, 2 * backoff), 64)
+            wait_until = sweeps + backoff
+            continue
+        if pending:
+            pending = False
+            backoff //= 2
         reset = len(F) > 0 and nf > nf_prev
         form = fprev is not None and not reset
-        sweeps += 1
         if reset:
             F, G = [], []
         elif form:
@@ -51,16 +68,18 @@ def anderson_mirror(sweep, K, m, every, reg, n3):
                 G.pop(0)
         fprev, gprev, nf_prev = f, gx, nf
         r = gx
-        if form and sweeps % every == 0:
+        if form and sweeps % every == 0 and sweeps >= wait_until:
             dF, dG = np.stack(F, 1), np.stack(G, 1)
             A = dF.T @ dF
             A += reg * np.trace(A) * np.eye(len(F))
             gam = np.linalg.solve(A, dF.T @ f)
             r = gx - dG @ gam
+            steps += 1
+            pending, nf_ref = True, nf
     return r
 
 
-@pytest.mark.parametrize("K", [2, 5, 12, 25])
+@pytest.mark.parametrize("K", [2, 5, 12, 25, 60])
 def test_iterates_match_the_restated_anderson_rule(oracle, K):
     g = N.generate_random_pairs(300, 1000, 0.2, 0.5, 1)
     o = oracle.Options()
@@ -77,9 +96,11 @@ def test_iterates_match_the_restated_anderson_rule(oracle, K):
     s = H.solve(_local_params(lo, tol=1e-300, max_iter=K, warm_start=1, accel_depth=3,
                               accel_every=2))
     assert s.iterations == K
-    ref = anderson_mirror(sweep, K, 3, 2, 1e-10, 3 * g.nc)
+    counts = {}
+    ref = anderson_mirror(sweep, K, 3, 2, 1e-10, 3 * g.nc, counts)
     r = H.reaction()
     assert np.linalg.norm(r - ref) <= 1e-9 * np.linalg.norm(ref)
+    assert (s.accel_steps, s.accel_rejects) == (counts["steps"], counts["rejects"])
     if K >= 5:
         assert s.accel_steps >= 1
 

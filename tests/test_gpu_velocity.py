@@ -72,3 +72,36 @@ def test_factors_are_checked_and_dropped_by_updates(lib):
     H.update(q=g.q)  # a new q: the factors (and v_free) may no longer match
     with pytest.raises(RuntimeError, match="-3"):
         H.solve(N.hip_params(velocity_form=1))
+    # attached again (new device buffers): the captured sweeps are rebuilt
+    H.set_bodies(g.nbodies, g.body_a, g.body_b, g.H, g.minv, g.vfree)
+    again = H.solve(N.hip_params(tol=1e-6, max_iter=3000, warm_start=1, velocity_form=1))
+    assert again.iterations == s.iterations
+    np.testing.assert_array_equal(H.reaction(), _fresh_reaction(g, velocity_form=1))
+
+
+def _fresh_reaction(g, **kw):
+    F = _with_bodies(g)
+    F.solve(N.hip_params(tol=1e-6, max_iter=3000, warm_start=1, **kw))
+    return F.reaction()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_anderson_steps_move_the_body_velocities_with_r(lib, precision):
+    """An Anderson step on r moves the body velocities by the same combination
+    of their own differences; after the solve, v must still equal
+    v_free + M^-1 H^T r, so u = H v matches W r + q up to rounding."""
+    g = N.generate_granular_pile(20000, 50000, 0.1, 0.9, 6)
+    H = _with_bodies(g)
+    prm = N.hip_params(tol=1e-6, max_iter=3000, warm_start=1, velocity_form=1, accel_depth=3,
+                       precision=precision, fp32_tol=1e-6)
+    s = H.solve(prm)
+    assert s.info == 0 and s.residual <= 1e-6 and s.accel_steps > 0
+    assert abs(H.compute_error() - s.residual) <= 1e-12 + 1e-9 * s.residual
+    m = H.solve(N.hip_params(tol=1e-6, max_iter=3000, warm_start=1, accel_depth=3,
+                             precision=precision, fp32_tol=1e-6))
+    print(f"\n[velocity] anderson precision {precision}: velocity {s.iterations} sweeps "
+          f"({s.accel_steps} steps), matrix {m.iterations} ({m.accel_steps} steps)")
+    assert s.iterations <= 1.25 * m.iterations + 10
+    # repeatable: the same solve again gives the same iterates
+    s2 = H.solve(prm)
+    assert s2.iterations == s.iterations
