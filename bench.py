@@ -4,10 +4,13 @@ Workload (default): the 1M-contact granular pile (400k spheres, mu ~ U[0.1,0.9],
 v_free ~ N(0,1) scaled by 0.1 on 10% of the bodies), solved to an FP64
 fc3d_compute_error <= 1e-6.  One step = one complete solve from a zero
 reaction with the data resident in HBM.  The reported method ("fast") is the
-coloured NSGS with FP32 sweeps accelerated by Anderson mixing on alternate
-sweeps, then FP64 sweeps to the FP64 residual.  Timed beside it on the same
-input: the plain FP64 coloured NSGS (reference options, zero start) and the
-fast method started from the learned warm start.
+coloured NSGS in the velocity form (local problems from the body velocities,
+W's factors H and M^-1 resident) with FP32 sweeps using the undamped
+Alart-Curnier Newton local solver, then FP64 sweeps with the reference's
+hybrid local solver until the FP64 residual meets the tolerance.  Timed
+beside it on the same input: the plain FP64 coloured NSGS (reference
+options, rows of W, zero start), the fast method on the rows of W, with
+safeguarded Anderson acceleration, and from the learned warm start.
 
 Prints one JSON line (rank 0).  `--gpus N` with N > 1 is refused: this build
 runs on one GPU.
@@ -35,14 +38,15 @@ WORKLOADS = {
 }
 
 
-# HBM traffic per FP64 sweep-kernel launch, from two rocprofv3 --pmc passes
-# (FETCH_SIZE, WRITE_SIZE) over `bench.py --steps 1 --warmup 0 --no-cpu`,
+# HBM traffic per launch of the reported method's dominant kernel
+# (k_sweep_vel<float>), from two rocprofv3 --pmc passes (FETCH_SIZE,
+# WRITE_SIZE) over `bench.py --steps 1 --warmup 0 --no-cpu --no-batch`,
 # scaled by the readings of a plain 1 GiB device copy (FETCH x2.000,
 # WRITE x1.000).  A profiler pass cannot run inside the timed bench, so the
 # figure is carried here with its source.
 PMC_TRAFFIC = {
-    "pile1m": {"bytes_per_launch": 100.49e6, "fetch": 97.80e6, "write": 2.69e6,
-               "source": "profiles/k_sweep_pile1m_hbm_traffic.md"},
+    "pile1m": {"bytes_per_launch": 34.553e6, "fetch": 28.376e6, "write": 6.177e6,
+               "source": "profiles/k_sweep_vel_pile1m_hbm_traffic.md"},
 }
 
 
@@ -119,7 +123,7 @@ def batch_leg(steps, warmup, cpu):
            "contacts": nc}
     for name, prm in (("fp64", N.hip_params(tol=1e-6, max_iter=1000)),
                       ("fp32_fp64", N.hip_params(tol=1e-6, max_iter=1000, precision=1,
-                                                 fp32_tol=1e-6))):
+                                                 fp32_tol=1e-6, fp32_local_solver=550))):
         zero = np.zeros(3 * nc)
         for _ in range(warmup):
             B.set_reaction(zero)
@@ -206,42 +210,55 @@ def main():
     log(f"[bench] warm-start network trained on {train_nc} contacts in "
         f"{time.perf_counter() - t0:.1f} s (loss {mlp_loss:.4f})")
 
+    # W's factors for the velocity form (resident, outside the timed region)
+    t0 = time.perf_counter()
+    H.set_bodies(g.nbodies, g.body_a, g.body_b, g.H, g.minv, g.vfree)
+    log(f"[bench] body factors attached in {time.perf_counter() - t0:.1f} s")
+
     plain = N.hip_params(tol=tol, max_iter=3000, sweeps_per_launch=16)
     fast_kw = dict(tol=tol, max_iter=3000, sweeps_per_launch=16, precision=1, fp32_tol=1e-6,
-                   accel_depth=3, accel_every=2)
-    fast = N.hip_params(warm_start=1, **fast_kw)
-    fast_mlp = N.hip_params(warm_start=2, **fast_kw)
+                   fp32_local_solver=550, warm_start=1)
+    legs = {
+        "fast": N.hip_params(velocity_form=1, **fast_kw),
+        "fast_rows_of_W": N.hip_params(**fast_kw),
+        "fast_with_anderson": N.hip_params(velocity_form=1, accel_depth=3, accel_every=2,
+                                           **fast_kw),
+        "fast_with_learned_start": N.hip_params(**dict(fast_kw, warm_start=2)),
+    }
     log("[bench] timing plain FP64 coloured NSGS")
     tp, devp, sp = time_method(H, plain, args.steps, args.warmup)
     log(f"[bench] plain: {tp / args.steps * 1e3:.1f} ms/solve, sweeps={sp.iterations}, "
         f"residual={sp.residual:.3e}")
-    log("[bench] timing FP32 Anderson sweeps + FP64 refinement")
-    tf, devf, sf = time_method(H, fast, args.steps, args.warmup)
-    log(f"[bench] fast: {tf / args.steps * 1e3:.1f} ms/solve, sweeps={sf.iterations} "
-        f"(fp32 {sf.fp32_sweeps}, anderson {sf.accel_steps}, restarts {sf.accel_restarts}), "
-        f"residual={sf.residual:.3e}")
-    log("[bench] timing the same from the learned warm start")
-    tn, devn, sn = time_method(H, fast_mlp, args.steps, args.warmup)
-    log(f"[bench] fast with learned start: {tn / args.steps * 1e3:.1f} ms/solve, "
-        f"sweeps={sn.iterations}, start used {sn.warm_start_used}, residual={sn.residual:.3e}")
+    res = {}
+    for name, prm in legs.items():
+        t, dev, st = time_method(H, prm, args.steps, args.warmup)
+        res[name] = (t, st)
+        log(f"[bench] {name}: {t / args.steps * 1e3:.1f} ms/solve, sweeps={st.iterations} "
+            f"(fp32 {st.fp32_sweeps}, anderson {st.accel_steps}/{st.accel_rejects} rejected), "
+            f"start used {st.warm_start_used}, residual={st.residual:.3e}")
+    tf, sf = res["fast"]
     ok_plain = sp.residual <= tol
     ok_fast = sf.residual <= tol
 
-    # sweep-kernel roofline: one instrumented plain solve; every colour launch
-    # carries a pair of HIP events stamped with the dispatch's own start and
-    # end on the library's stream
+    # roofline of the dominant kernel, the FP32 velocity-form colour pass: one
+    # instrumented fast solve; every colour launch carries a pair of HIP
+    # events stamped with the dispatch's own start and end on the library's
+    # stream, and the FP32 launches are summed apart
     H.set_reaction(np.zeros(3 * H.nc))
-    si = H.solve(N.hip_params(tol=tol, max_iter=2000, instrument=1))
-    achieved = si.sweep_kernel_bytes / (si.sweep_kernel_ms * 1e-3) / 1e9
+    si = H.solve(N.hip_params(**dict(fast_kw, velocity_form=1, instrument=1)))
+    achieved = si.sweep_kernel_bytes_fp32 / (si.sweep_kernel_ms_fp32 * 1e-3) / 1e9
     pmc = PMC_TRAFFIC.get(args.workload)
+    nl = max(si.sweep_kernel_launches_fp32, 1)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                 "frac": achieved / 8000.0,
                 "traffic": pmc["bytes_per_launch"] if pmc else None,
                 "traffic_source": pmc["source"] if pmc else None,
-                "kernel": "k_sweep<double> (one colour pass)",
-                "launch_avg_us": si.sweep_kernel_ms * 1e3 / max(si.sweep_kernel_launches, 1),
-                "bytes_per_launch_avg": si.sweep_kernel_bytes / max(si.sweep_kernel_launches, 1)}
-    log(f"[bench] sweep kernel: {achieved:.1f} GB/s over {si.sweep_kernel_launches} launches")
+                "kernel": "k_sweep_vel<float> (one colour pass, velocity form)",
+                "launches": int(si.sweep_kernel_launches_fp32),
+                "launch_avg_us": si.sweep_kernel_ms_fp32 * 1e3 / nl,
+                "bytes_per_launch_avg": si.sweep_kernel_bytes_fp32 / nl}
+    log(f"[bench] sweep kernel: {achieved:.1f} GB/s over {si.sweep_kernel_launches_fp32} "
+        f"FP32 launches, {roofline['launch_avg_us']:.2f} us each")
 
     cpu = None
     if not args.no_cpu:
@@ -265,6 +282,22 @@ def main():
     nc = g.nc
     value = nc * args.steps / tf if ok_fast else 0.0
     plain_value = nc * args.steps / tp if ok_plain else 0.0
+
+    def leg(name):
+        t, st = res[name]
+        d = {"value": nc * args.steps / t if st.residual <= tol else 0.0,
+             "ms_per_step": t / args.steps * 1e3, "sweeps": st.iterations,
+             "fp32_sweeps": st.fp32_sweeps, "residual": st.residual}
+        if st.accel_steps:
+            d.update(anderson_steps=st.accel_steps, anderson_rejected=st.accel_rejects)
+        return d
+
+    learned = leg("fast_with_learned_start")
+    tn, sn = res["fast_with_learned_start"]
+    learned.update(
+        warm_start_used={1: "zero (fallback)", 2: "learned"}.get(sn.warm_start_used),
+        residual_after_first_sweep={"zero": sn.start_residual_fallback,
+                                    "learned": sn.start_residual_learned})
     out = {
         "metric": METRIC, "value": value, "unit": "contact-solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": tf / args.steps * 1e3,
@@ -272,21 +305,16 @@ def main():
         "dtype": "f32 sweeps + f64 residual/refinement", "data": "synthetic",
         "config": {"workload": w["desc"], "contacts": nc, "bodies": w["nbodies"],
                    "nnz_blocks": int(g.W.nnzb), "colours": H.num_colours, "tol": tol,
-                   "method": "coloured NSGS from a zero start: FP32 sweeps with Anderson(3) "
-                             "on alternate sweeps, then FP64 sweeps to the FP64 residual; the "
-                             "learned warm start is timed beside it"},
+                   "method": "coloured NSGS from a zero start in the velocity form: FP32 "
+                             "sweeps with the undamped AC Newton local solver, then FP64 "
+                             "sweeps with the reference hybrid local solver to the FP64 "
+                             "residual"},
         "plain_fp64": {"value": plain_value, "ms_per_step": tp / args.steps * 1e3,
                        "sweeps": sp.iterations, "residual": sp.residual},
-        "fast": {"sweeps": sf.iterations, "fp32_sweeps": sf.fp32_sweeps,
-                 "anderson_steps": sf.accel_steps, "anderson_restarts": sf.accel_restarts,
-                 "residual": sf.residual},
-        "fast_with_learned_start": {
-            "value": nc * args.steps / tn if sn.residual <= tol else 0.0,
-            "ms_per_step": tn / args.steps * 1e3, "sweeps": sn.iterations,
-            "warm_start_used": {1: "zero (fallback)", 2: "learned"}.get(sn.warm_start_used),
-            "residual_after_first_sweep": {"zero": sn.start_residual_fallback,
-                                           "learned": sn.start_residual_learned},
-            "residual": sn.residual},
+        "fast": leg("fast"),
+        "fast_rows_of_W": leg("fast_rows_of_W"),
+        "fast_with_anderson": leg("fast_with_anderson"),
+        "fast_with_learned_start": learned,
         "warm_start_training": {"contacts": train_nc, "final_loss": mlp_loss},
         "speedup_vs_plain": (value / plain_value) if plain_value else None,
         "roofline": roofline, "cpu_baseline": cpu, "batch": batch, "assembly": assembly,

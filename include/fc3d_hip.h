@@ -58,7 +58,7 @@ typedef struct fc3d_hip_params {
                                 kept only if its FP64 residual after one sweep is
                                 below the zero start's; 3: the same against the
                                 resident reaction (previous time step)          */
-  int schedule;              /* 0: autotuned; 1..9 explicit (7: single launch)    */
+  int schedule;              /* 0: autotuned; 1..10 explicit (7: single launch)   */
   int sweeps_per_launch;     /* sweeps captured per graph replay (>=1)            */
   double fp32_tol;           /* FP32 phase hands over when its light error < this */
   double fp32_local_tol;     /* local tolerance used by FP32 sweeps               */
@@ -74,6 +74,11 @@ typedef struct fc3d_hip_params {
    * kept current, instead of the row of W; the same iteration in exact
    * arithmetic.  Convergence is still judged on the FP64 residual with W. */
   int velocity_form;         /* 0: rows of W (default); 1: velocity form          */
+  /* Local solver of the FP32 sweeps (precision = 1): 0 keeps local_solver;
+   * 550 (undamped AC Newton) is the cheapest that converges on piles.  The
+   * FP64 sweeps always use local_solver, and the result is judged on the
+   * FP64 residual. */
+  int fp32_local_solver;
 } fc3d_hip_params;
 
 typedef struct fc3d_hip_stats {
@@ -97,6 +102,10 @@ typedef struct fc3d_hip_stats {
   int accel_restarts;         /* history restarts                                */
   int accel_rejects;          /* steps undone by the safeguard (each costs the   *
                                * sweep that judged it)                           */
+  /* instrumented: the FP32-sweep share of the three sweep_kernel_* figures */
+  double sweep_kernel_ms_fp32;
+  int64_t sweep_kernel_launches_fp32;
+  double sweep_kernel_bytes_fp32;
 } fc3d_hip_stats;
 
 FC3D_API int fc3d_hip_device_count(void);

@@ -75,7 +75,7 @@ class HipParams(C.Structure):
                 ("schedule", C.c_int), ("sweeps_per_launch", C.c_int), ("fp32_tol", C.c_double),
                 ("fp32_local_tol", C.c_double), ("instrument", C.c_int),
                 ("accel_depth", C.c_int), ("accel_every", C.c_int), ("accel_reg", C.c_double),
-                ("velocity_form", C.c_int)]
+                ("velocity_form", C.c_int), ("fp32_local_solver", C.c_int)]
 
 
 class HipStats(C.Structure):
@@ -86,7 +86,9 @@ class HipStats(C.Structure):
                 ("sweep_kernel_bytes", C.c_double), ("schedule", C.c_int),
                 ("start_residual_fallback", C.c_double), ("start_residual_learned", C.c_double),
                 ("accel_steps", C.c_int), ("accel_restarts", C.c_int),
-                ("accel_rejects", C.c_int)]
+                ("accel_rejects", C.c_int), ("sweep_kernel_ms_fp32", C.c_double),
+                ("sweep_kernel_launches_fp32", C.c_int64),
+                ("sweep_kernel_bytes_fp32", C.c_double)]
 
 
 class HipBatchStats(C.Structure):

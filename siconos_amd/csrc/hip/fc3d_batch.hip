@@ -502,6 +502,11 @@ int fc3d_hip_batch_solve(fc3d_hip_batch* b, const fc3d_hip_params* p, fc3d_hip_b
     set_error("fc3d_hip_batch_solve: local solver %d not supported", ls);
     return NUMERICS_ERR_INVALID_SOLVER;
   }
+  const int ls32 = p->fp32_local_solver;
+  if (!(ls32 == 0 || ls32 == 565 || ls32 == 551 || ls32 == 550 || ls32 == 553 || ls32 == 552)) {
+    set_error("fc3d_hip_batch_solve: FP32 local solver %d not supported", ls32);
+    return NUMERICS_ERR_INVALID_SOLVER;
+  }
   if (p->error_evaluation < 0 || p->error_evaluation > 3 || p->warm_start >= 2) {
     set_error("fc3d_hip_batch_solve: unsupported evaluation mode or warm start");
     return NUMERICS_ERR_INVALID_ARGUMENT;
@@ -522,6 +527,8 @@ int fc3d_hip_batch_solve(fc3d_hip_batch* b, const fc3d_hip_params* p, fc3d_hip_b
   d.lp.hybrid_strategy = p->hybrid_strategy;
   d.lp.hybrid_max_loop = p->hybrid_max_loop;
   d.lp.hybrid_max_iter = p->hybrid_max_iter;
+  d.lp32 = d.lp;
+  if (p->fp32_local_solver) d.lp32.solver = p->fp32_local_solver;
   d.tol = p->tol;
   d.local_tol = p->local_tol;
   d.fp32_local_tol = p->fp32_local_tol;

@@ -2,8 +2,9 @@
 //
 // Every function restates one reference function; the operation order is the
 // reference's, so that a build with -ffp-contract=off gives the same FP64
-// results as the reference C.  T is the sweep precision (double or float);
-// the 3x3 Newton factorisation always runs in double.
+// results as the reference C.  T is the sweep precision (double or float).
+// The 3x3 Newton factorisation runs in double for T = double and, for
+// T = float, in float unless the matrix is near-degenerate.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -36,11 +37,16 @@ __device__ __forceinline__ void mm3x3(const T* a, const T* b, T* c) {
       c[i + 3 * j] = a[i + 0] * b[0 + 3 * j] + a[i + 3] * b[1 + 3 * j] + a[i + 6] * b[2 + 3 * j];
 }
 
-// op3x3.h:868-981 — partial-pivot 3x3 solve, 0 on success
-__device__ __forceinline__ int solve_3x3_gepp(const double* a, double* b) {
-  double lp0, lp1, lp2, lm1, lm2, ln1, ln2, bl, bm, bn, f1, f2, s0, s1, s2, alp0;
-  const double a0 = a[0], a1 = a[1], a2 = a[2];
-  const double aa0 = fabs(a0), aa1 = fabs(a1), aa2 = fabs(a2);
+// op3x3.h:868-981 — partial-pivot 3x3 solve, 0 on success.  The FP64 sweeps
+// call it in double, as the reference does; the FP32 sweeps call it in float
+// first (see nsn_newton).  The pivot test keeps the reference's absolute
+// DBL_EPSILON threshold in both.  With `minpiv`, the smallest pivot magnitude
+// used is returned there.
+template <typename G>
+__device__ __forceinline__ int solve_3x3_gepp(const G* a, G* b, G* minpiv = nullptr) {
+  G lp0, lp1, lp2, lm1, lm2, ln1, ln2, bl, bm, bn, f1, f2, s0, s1, s2, alp0;
+  const G a0 = a[0], a1 = a[1], a2 = a[2];
+  const G aa0 = fabs(a0), aa1 = fabs(a1), aa2 = fabs(a2);
   const int p1 = aa0 >= aa1 ? (aa0 >= aa2 ? 0 : 2) : (aa1 >= aa2 ? 1 : 2);
   if (p1 == 0) {
     f1 = a1 / a0; f2 = a2 / a0; lp0 = a0; alp0 = fabs(a0); lp1 = a[3]; lp2 = a[6];
@@ -55,19 +61,23 @@ __device__ __forceinline__ int solve_3x3_gepp(const double* a, double* b) {
     lm1 = a[3] - f1 * lp1; lm2 = a[6] - f1 * lp2; ln1 = a[4] - f2 * lp1; ln2 = a[7] - f2 * lp2;
     bl = b[2]; bm = b[0] - f1 * bl; bn = b[1] - f2 * bl;
   }
-  if (alp0 <= DBL_EPSILON) return 1;
-  const double alm1 = fabs(lm1), aln1 = fabs(ln1);
+  if (alp0 <= G(DBL_EPSILON)) return 1;
+  const G alm1 = fabs(lm1), aln1 = fabs(ln1);
+  G p2;
   if (alm1 >= aln1) {
-    if (alm1 < DBL_EPSILON) return 1;
+    if (alm1 < G(DBL_EPSILON)) return 1;
     f1 = ln1 / lm1;
-    s2 = (bn - f1 * bm) / (ln2 - f1 * lm2);
+    p2 = ln2 - f1 * lm2;
+    s2 = (bn - f1 * bm) / p2;
     s1 = (bm - lm2 * s2) / lm1;
   } else {
-    if (aln1 < DBL_EPSILON) return 1;
+    if (aln1 < G(DBL_EPSILON)) return 1;
     f1 = lm1 / ln1;
-    s2 = (bm - f1 * bn) / (lm2 - f1 * ln2);
+    p2 = lm2 - f1 * ln2;
+    s2 = (bm - f1 * bn) / p2;
     s1 = (bn - ln2 * s2) / ln1;
   }
+  if (minpiv) *minpiv = fmin(fmin(alp0, fmax(alm1, aln1)), fabs(p2));
   s0 = (bl - s1 * lp1 - s2 * lp2) / lp0;
   b[0] = s0; b[1] = s1; b[2] = s2;
   return 0;
@@ -363,15 +373,35 @@ __device__ int nsn_newton(const LocalProblem<T>& lp, T* r, const LocalParams& P,
     ac_function<T, true>(P.formulation, r, v, lp.mu, rho, F, A, B);
     T AW[9];
     mm3x3<T>(A, lp.W, AW);
-    double M[9], d[3];
+    T M[9], d[3];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
       T x = AW[k] + B[k];
       x = x * T(-1);
-      M[k] = double(x);
+      M[k] = x;
     }
-    d[0] = double(F[0]); d[1] = double(F[1]); d[2] = double(F[2]);
-    const int info3 = solve_3x3_gepp(M, d);
+    d[0] = F[0]; d[1] = F[1]; d[2] = F[2];
+    int info3;
+    if (sizeof(T) == 8) {
+      info3 = solve_3x3_gepp<T>(M, d);
+    } else {
+      // FP32 factorisation, redone in FP64 for a near-degenerate matrix
+      // (smallest pivot below 2^-12 of the largest entry)
+      T scale = T(0);
+#pragma unroll
+      for (int k = 0; k < 9; k++) scale = fmax(scale, fabs(M[k]));
+      T Mf[9], piv = T(0);
+#pragma unroll
+      for (int k = 0; k < 9; k++) Mf[k] = M[k];
+      info3 = solve_3x3_gepp<T>(Mf, d, &piv);
+      if (info3 || !(piv > T(0x1p-12) * scale)) {
+        double M64[9], d64[3] = {double(F[0]), double(F[1]), double(F[2])};
+#pragma unroll
+        for (int k = 0; k < 9; k++) M64[k] = double(M[k]);
+        info3 = solve_3x3_gepp<double>(M64, d64);
+        d[0] = T(d64[0]); d[1] = T(d64[1]); d[2] = T(d64[2]);
+      }
+    }
     if (info3) {
       dR[0] = T(0); dR[1] = T(0); dR[2] = T(0);
     } else {

@@ -1202,6 +1202,8 @@ static DevParams make_params(const fc3d_hip_params* p) {
   d.lp.hybrid_strategy = p->hybrid_strategy;
   d.lp.hybrid_max_loop = p->hybrid_max_loop;
   d.lp.hybrid_max_iter = p->hybrid_max_iter;
+  d.lp32 = d.lp;
+  if (p->fp32_local_solver) d.lp32.solver = p->fp32_local_solver;
   d.tol = p->tol;
   d.local_tol = p->local_tol;
   d.fp32_local_tol = p->fp32_local_tol;
@@ -1617,18 +1619,20 @@ int fc3d_hip_problem_set_bodies(fc3d_hip_problem* h, int nbodies, const int* bod
     return rc;
   h->nbodies = nbodies;
   h->minv_diag = minv_diagonal ? 1 : 0;
-  // algorithmic bytes per contact of one velocity-form colour pass: H, W_cc,
-  // mu, r (read, write), PLI step (read, write), two body ids, two bodies'
-  // velocities (read, write) and M^-1
+  // algorithmic bytes per contact of one velocity-form colour pass, in
+  // values of the sweep precision: H (36), W_cc (9), mu (1), r read and
+  // written (4 + 4, padded vec4), PLI step read and written (2); per body
+  // side present: its velocity read and written (8 + 8, two padded vec4) and
+  // M^-1 (6 or 36); plus len and the two body ids (12 bytes)
   const double mb = minv_diagonal ? 6.0 : 36.0;
-  const double per64 = 8.0 * (36 + 9 + 1 + 2 + 4 + 4 + 2 * 2 * 8 + 2 * mb) + 8.0;
-  const double per32 = 4.0 * (36 + 9 + 1 + 2 + 4 + 4 + 2 * 2 * 8 + 2 * mb) + 8.0;
   h->vel_bytes64.assign((size_t)h->ncolours, 0.0);
   h->vel_bytes32.assign((size_t)h->ncolours, 0.0);
   for (int c = 0; c < nc; c++) {
     const int col = h->colour[(size_t)c];
-    h->vel_bytes64[(size_t)col] += per64;
-    h->vel_bytes32[(size_t)col] += per32;
+    const double sides = (body_a[c] >= 0 ? 1.0 : 0.0) + (body_b[c] >= 0 ? 1.0 : 0.0);
+    const double vals = 36 + 9 + 1 + 8 + 2 + sides * (16 + mb);
+    h->vel_bytes64[(size_t)col] += 8.0 * vals + 12.0;
+    h->vel_bytes32[(size_t)col] += 4.0 * vals + 12.0;
   }
   return 0;
 }
@@ -1684,12 +1688,17 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
     set_error("fc3d_hip_solve: local solver %d not supported", ls);
     return NUMERICS_ERR_INVALID_SOLVER;
   }
+  const int ls32 = p->fp32_local_solver;
+  if (!(ls32 == 0 || ls32 == 565 || ls32 == 551 || ls32 == 550 || ls32 == 553 || ls32 == 552)) {
+    set_error("fc3d_hip_solve: FP32 local solver %d not supported", ls32);
+    return NUMERICS_ERR_INVALID_SOLVER;
+  }
   if (p->error_evaluation < 0 || p->error_evaluation > 3) {
     set_error("fc3d_hip_solve: unknown error evaluation %d", p->error_evaluation);
     return NUMERICS_ERR_INVALID_ARGUMENT;
   }
   if (p->rho_strategy != 0 && p->rho_strategy != 2 && p->rho_strategy != 3 &&
-      (ls == 565 || ls == 551 || ls == 550)) {
+      (ls == 565 || ls == 551 || ls == 550 || ls32 == 565 || ls32 == 551 || ls32 == 550)) {
     set_error("fc3d_hip_solve: rho strategy %d not supported", p->rho_strategy);
     return NUMERICS_ERR_NOT_IMPLEMENTED;
   }
@@ -1841,9 +1850,15 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
         for (int c = 0; c < timed_launches(h, sched[phase]); c++) {
           float ms = 0.f;
           hipEventElapsedTime(&ms, kev[2 * (size_t)c], kev[2 * (size_t)c + 1]);
+          const double by = timed_bytes(h, phase, sched[phase], c);
           s->sweep_kernel_ms += ms;
           s->sweep_kernel_launches += 1;
-          s->sweep_kernel_bytes += timed_bytes(h, phase, sched[phase], c);
+          s->sweep_kernel_bytes += by;
+          if (phase) {
+            s->sweep_kernel_ms_fp32 += ms;
+            s->sweep_kernel_launches_fp32 += 1;
+            s->sweep_kernel_bytes_fp32 += by;
+          }
         }
       } else if (no_graph) {
         for (int k = 0; k < chunk; k++) enqueue_sweep(h, phase, sched[phase], nullptr, 0);

@@ -12,6 +12,7 @@ template <> struct vec4<float> { using type = float4; };
 
 struct DevParams {
   LocalParams lp;
+  LocalParams lp32;     // local solver of the FP32 sweeps
   double tol;           // user tolerance
   double local_tol;     // local tolerance at solve start (FP64 phase)
   double fp32_local_tol;
@@ -187,7 +188,7 @@ __device__ __forceinline__ void finish_slot(const SweepPtrs<T>& P, const DevPara
   T rho[3];
   compute_rho<T>(prm->rho_strategy, T(prm->rho_constant), lp.W, rho);
   T res = T(0);
-  local_solve<T>(lp, lr, prm->lp, tol, &pli, rho, &res);
+  local_solve<T>(lp, lr, sizeof(T) == 8 ? prm->lp : prm->lp32, tol, &pli, rho, &res);
   if (prm->relaxation) {
     const T om = T(prm->omega);
 #pragma unroll

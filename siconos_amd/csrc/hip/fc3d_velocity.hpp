@@ -88,27 +88,35 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void k_sweep_vel(Sweep
                                                       const DevParams* __restrict__ prm,
                                                       const SolveState* __restrict__ st,
                                                       int s_begin, int s_end, int want_phase) {
-  if (st->stop || st->phase != want_phase) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int s = __builtin_amdgcn_readfirstlane(s_begin + xcd_block() * 4 + wave);
   if (s >= s_end) return;
   const int slot = s * 64 + lane;
   double light = 0.0, nr2 = 0.0;
-  if (P.len[slot] >= 0) {
+  // Every load of the slot is issued before the stop flag is tested, so the
+  // flag's latency overlaps theirs (all addresses are in bounds: padding
+  // slots read body 0).  The empty asm makes the loaded values live before
+  // the test, which keeps the compiler from sinking the loads below it.
+  const int stop = st->stop, phase = st->phase;
+  const int len = P.len[slot];
+  SlotInputs<T> in;
+  load_slot<T>(P, s, lane, &in);
+  const int kb[2] = {B.ba[slot], B.bb[slot]};
+  T h[36];
+#pragma unroll
+  for (int e = 0; e < 36; e++) h[e] = B.H[((size_t)s * 36 + e) * 64 + lane];
+  typename vec4<T>::type va[2][2];
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    const size_t k = kb[side] >= 0 ? (size_t)kb[side] : 0;
+    va[side][0] = B.V[2 * k];
+    va[side][1] = B.V[2 * k + 1];
+  }
+  asm volatile("" ::"v"(len), "v"(in.lp.W[0]), "v"(in.lp.mu), "v"(in.ro.x), "v"(in.pli),
+               "v"(h[0]), "v"(h[35]), "v"(va[0][0].x), "v"(va[1][1].x), "s"(stop), "s"(phase));
+  if (stop || phase != want_phase) return;
+  if (len >= 0) {
     const T tol = T(want_phase ? prm->fp32_local_tol : st->local_tol);
-    SlotInputs<T> in;
-    load_slot<T>(P, s, lane, &in);
-    const int kb[2] = {B.ba[slot], B.bb[slot]};
-    T h[36];
-#pragma unroll
-    for (int e = 0; e < 36; e++) h[e] = B.H[((size_t)s * 36 + e) * 64 + lane];
-    typename vec4<T>::type va[2][2];
-#pragma unroll
-    for (int side = 0; side < 2; side++)
-      if (kb[side] >= 0) {
-        va[side][0] = B.V[2 * (size_t)kb[side]];
-        va[side][1] = B.V[2 * (size_t)kb[side] + 1];
-      }
     // u = H_a v_a + H_b v_b, then q_loc = u - W_cc r_old
     T u[3] = {T(0), T(0), T(0)};
 #pragma unroll

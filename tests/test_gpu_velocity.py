@@ -105,3 +105,42 @@ def test_anderson_steps_move_the_body_velocities_with_r(lib, precision):
     # repeatable: the same solve again gives the same iterates
     s2 = H.solve(prm)
     assert s2.iterations == s.iterations
+
+
+@pytest.mark.parametrize("velocity_form", [0, 1])
+def test_fp32_sweeps_with_their_own_local_solver(lib, velocity_form):
+    """fp32_local_solver changes the local solver of the FP32 sweeps only: the
+    FP64 sweeps keep the reference's hybrid solver and the FP64 tolerance
+    still holds; an FP64-only solve ignores the option bit for bit."""
+    g = N.generate_granular_pile(20000, 50000, 0.1, 0.9, 5)
+    H = _with_bodies(g)
+    base = dict(tol=1e-6, max_iter=3000, warm_start=1, velocity_form=velocity_form)
+    ref = H.solve(N.hip_params(**base))
+    r_ref = H.reaction()
+    same = H.solve(N.hip_params(fp32_local_solver=550, **base))
+    assert same.iterations == ref.iterations
+    np.testing.assert_array_equal(H.reaction(), r_ref)
+    for ls in (550, 551):
+        s = H.solve(N.hip_params(precision=1, fp32_tol=1e-6, fp32_local_solver=ls, **base))
+        assert s.info == 0 and s.residual <= 1e-6 and s.fp32_sweeps > 0
+        assert abs(H.compute_error() - s.residual) <= 1e-12 + 1e-9 * s.residual
+        assert s.iterations <= 1.1 * ref.iterations + 10
+    with pytest.raises(RuntimeError, match="-4"):
+        H.solve(N.hip_params(precision=1, fp32_local_solver=123, **base))
+
+
+def test_driver_maps_the_fp32_local_solver(lib):
+    g = N.generate_granular_pile(4000, 10000, 0.1, 0.9, 7)
+    prob = N.FrictionContactProblem(3, g.W, g.q, g.mu)
+    o = N.SolverOptions(N.SICONOS_FRICTION_3D_NSGS_HIP)
+    o.dparam[N.SICONOS_DPARAM_TOL] = 1e-6
+    o.iparam[N.SICONOS_IPARAM_MAX_ITER] = 3000
+    o.iparam[20] = 1  # FP32 sweeps
+    o.dparam[26] = 550.0
+    r = np.zeros(3 * g.nc)
+    u = np.zeros(3 * g.nc)
+    assert N.fc3d_driver(prob, r, u, o) == 0
+    assert o.dparam[N.SICONOS_DPARAM_RESIDU] <= 1e-6
+    o.dparam[26] = 123.0
+    r[:] = 0.0
+    assert N.fc3d_driver(prob, r, u, o) != 0
