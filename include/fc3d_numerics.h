@@ -87,7 +87,36 @@ typedef struct SparseBlockStructuredMatrix {
   NumericsDataVersion version;
 } SparseBlockStructuredMatrix;
 
-struct NumericsSparseMatrix; /* compressed storage: not handled on this path */
+/* Compressed storage.  CSparseMatrix keeps the layout of CXSparse's
+ * cs_dl_sparse (CSparseMatrix.h:37 with CS_LONG): p holds n + 1 column
+ * pointers (CSC) or the column index of each entry (triplet), i the row
+ * indices; nz = -1 marks CSC, -2 CSR, >= 0 a triplet of nz entries.
+ * NumericsSparseMatrix keeps NumericsSparseMatrix.h:95-112.  On this path
+ * an NM_SPARSE matrix is read from a .dat file (NSM_new_from_file,
+ * NumericsSparseMatrix.c:555-583) and used by the fc2d solver. */
+typedef struct CSparseMatrix {
+  int64_t nzmax;
+  int64_t m;
+  int64_t n;
+  int64_t* p;
+  int64_t* i;
+  double* x;
+  int64_t nz;
+} CSparseMatrix;
+
+typedef enum { NSM_UNKNOWN, NSM_TRIPLET, NSM_CSC, NSM_CSR, NSM_HALF_TRIPLET } NSM_t;
+
+typedef struct NumericsSparseMatrix {
+  CSparseMatrix* triplet;
+  CSparseMatrix* half_triplet;
+  CSparseMatrix* csc;
+  CSparseMatrix* trans_csc;
+  CSparseMatrix* csr;
+  int64_t* diag_indx;
+  NSM_t origin;
+  void* linearSolverParams; /* not used on this path */
+  NumericsDataVersion versions[5];
+} NumericsSparseMatrix;
 
 typedef struct NumericsMatrix {
   NM_types storageType;
@@ -95,7 +124,7 @@ typedef struct NumericsMatrix {
   int size1;
   double* matrix0;                      /* dense, column-major             */
   SparseBlockStructuredMatrix* matrix1; /* sparse block                    */
-  struct NumericsSparseMatrix* matrix2; /* unsupported here (NULL)         */
+  NumericsSparseMatrix* matrix2;        /* compressed (NM_SPARSE)          */
   void* internalData;
   NumericsDataVersion version;
   struct NumericsMatrix* destructible;
@@ -339,10 +368,19 @@ FC3D_API SolverOptions* solver_options_get_internal_solver(SolverOptions* option
 /* ------------------------------------------------------------------------ */
 /* Matrices (host) — NumericsMatrix.c, SparseBlockMatrix.c                  */
 /* ------------------------------------------------------------------------ */
-/* NumericsMatrix.c NM_create_from_data: wraps data (SBM* or double*) */
+/* NumericsMatrix.c NM_create_from_data: wraps data (double* for NM_DENSE,
+ * SBM* for NM_SPARSE_BLOCK, NumericsSparseMatrix* for NM_SPARSE) */
 FC3D_API NumericsMatrix* NM_create_from_data(int storageType, int size0, int size1, void* data);
 /* NumericsMatrix.c NM_free: releases the matrix and its storage */
 FC3D_API NumericsMatrix* NM_free(NumericsMatrix* m);
+/* NumericsSparseMatrix.c:555-583 and CSparseMatrix.c:651-790: an NM_SPARSE
+ * matrix as NM_new_from_file reads it (origin, then a cs_print listing) */
+FC3D_API NumericsSparseMatrix* NSM_new_from_file(FILE* file);
+FC3D_API int NSM_write_in_file(const NumericsSparseMatrix* A, FILE* file);
+FC3D_API NumericsSparseMatrix* NSM_free(NumericsSparseMatrix* A);
+/* NumericsMatrix.c NM_csc: the CSC form of an NM_SPARSE matrix (compressed
+ * from the triplet on first use, as NM_csc does with cs_compress) */
+FC3D_API CSparseMatrix* NM_csc(NumericsMatrix* A);
 /* SparseBlockMatrix.c SBM_new / SBM_free(SBM_FREE_ALL) */
 FC3D_API SparseBlockStructuredMatrix* SBM_new(void);
 FC3D_API void SBM_free_all(SparseBlockStructuredMatrix* m);

@@ -131,6 +131,29 @@ ORACLE_API int oracle_solve_many(int nprob, const int* nc, const int64_t* row_pt
                                  const double* mu, double* reaction, double tol, int max_iter,
                                  int nthreads, int* iters, double* residuals);
 
+/* ---- fc2d (fc2d_oracle.c) ---------------------------------------------- */
+/* Reads a dimension-2 .dat problem (dense, SBM of 2x2 blocks or CSparse);
+ * NULL on failure.  The handle is released with oracle_fc2d_free. */
+ORACLE_API void* oracle_fc2d_read(const char* path);
+ORACLE_API void oracle_fc2d_free(void* h);
+/* out[0] contacts, out[1] storage type, out[2] sparse origin */
+ORACLE_API int oracle_fc2d_info(void* h, int* out);
+/* dense column-major copy of W (2nc x 2nc), q and mu */
+ORACLE_API int oracle_fc2d_dense(void* h, double* W, double* q, double* mu);
+/* fc2d_nsgs.c:97-113: q_loc of contact c at reaction x */
+ORACLE_API int oracle_fc2d_local_q(void* h, int c, double* x, double* y);
+/* fc2d_nsgs.c:247-284 */
+ORACLE_API void oracle_fc2d_local_solve(const double* W, double D, const double* q, double mu,
+                                        double* P);
+/* fc2d_compute_error.c:104-148 with norm = |q| */
+ORACLE_API int oracle_fc2d_compute_error(void* h, const double* z, double* w, double* error);
+/* fc2d_nsgs.c:297-538; mode is a SICONOS_NSGS_ERROR_EVALUATION_* value;
+ * returns info, out = {iterations, residual} */
+ORACLE_API int oracle_fc2d_nsgs(void* h, double* reaction, double* velocity, double tol,
+                                int itermax, int mode, const int* order, double* out);
+/* 1: slide directions swapped in the local solve (a planted mistake) */
+ORACLE_API void oracle_fc2d_set_mutation(int m);
+
 #ifdef __cplusplus
 }
 #endif

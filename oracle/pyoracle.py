@@ -57,6 +57,16 @@ def load():
         "oracle_problem_free": (None, [C.c_void_p]),
         "oracle_solve_many": (C.c_int, [C.c_int, ip, lp, lp, lp, lp, dp, dp, dp, dp, C.c_double,
                                         C.c_int, C.c_int, ip, dp]),
+        "oracle_fc2d_read": (C.c_void_p, [C.c_char_p]),
+        "oracle_fc2d_free": (None, [C.c_void_p]),
+        "oracle_fc2d_info": (C.c_int, [C.c_void_p, ip]),
+        "oracle_fc2d_dense": (C.c_int, [C.c_void_p, dp, dp, dp]),
+        "oracle_fc2d_local_q": (C.c_int, [C.c_void_p, C.c_int, dp, dp]),
+        "oracle_fc2d_local_solve": (None, [dp, C.c_double, dp, C.c_double, dp]),
+        "oracle_fc2d_compute_error": (C.c_int, [C.c_void_p, dp, dp, dp]),
+        "oracle_fc2d_nsgs": (C.c_int, [C.c_void_p, dp, dp, C.c_double, C.c_int, C.c_int, ip,
+                                       dp]),
+        "oracle_fc2d_set_mutation": (None, [C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -211,3 +221,69 @@ def solve_many(problems, tol, max_iter, nthreads):
 
 def mutation(m):
     load().oracle_set_mutation(m)
+
+
+class Fc2dProblem:
+    """A dimension-2 .dat problem read by the oracle's own parser."""
+
+    def __init__(self, path):
+        self.lib = load()
+        self.h = self.lib.oracle_fc2d_read(path.encode())
+        if not self.h:
+            raise ValueError(f"oracle_fc2d_read failed on {path}")
+        info = np.zeros(3, dtype=np.int32)
+        self.lib.oracle_fc2d_info(self.h, info.ctypes.data_as(ip))
+        self.nc, self.storage, self.origin = (int(x) for x in info)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.oracle_fc2d_free(self.h)
+            self.h = None
+
+    def dense(self):
+        n = 2 * self.nc
+        W = np.zeros((n, n), order="F")
+        q = np.zeros(n)
+        mu = np.zeros(self.nc)
+        self.lib.oracle_fc2d_dense(self.h, W.ctypes.data_as(dp), d(q), d(mu))
+        return W, q, mu
+
+    def local_q(self, c, r):
+        x = np.array(r, dtype=np.float64)
+        y = np.zeros(2)
+        self.lib.oracle_fc2d_local_q(self.h, c, d(x), d(y))
+        return y
+
+    def compute_error(self, r):
+        z = np.ascontiguousarray(r, dtype=np.float64)
+        w = np.zeros(2 * self.nc)
+        e = C.c_double()
+        self.lib.oracle_fc2d_compute_error(self.h, d(z), d(w), C.byref(e))
+        return e.value, w
+
+    def nsgs(self, tol=1e-5, max_iter=10000, mode=2, order=None, reaction=None):
+        """oracle_fc2d_nsgs; mode: 1 LIGHT, 2 LIGHT_WITH_FULL_FINAL (others
+        are not evaluated); returns (info, r, u, iterations, residual)."""
+        r = np.zeros(2 * self.nc) if reaction is None else np.array(reaction, dtype=np.float64)
+        u = np.zeros(2 * self.nc)
+        out = np.zeros(2)
+        op = None
+        if order is not None:
+            order = np.ascontiguousarray(order, dtype=np.int32)
+            op = order.ctypes.data_as(ip)
+        info = self.lib.oracle_fc2d_nsgs(self.h, d(r), d(u), tol, max_iter, mode, op, d(out))
+        return info, r, u, int(out[0]), float(out[1])
+
+
+def fc2d_local_solve(W, q, mu):
+    W = np.ascontiguousarray(W, dtype=np.float64).ravel(order="F") if np.ndim(W) == 2 else \
+        np.ascontiguousarray(W, dtype=np.float64)
+    D = W[0] * W[3] - W[1] * W[2]
+    P = np.zeros(2)
+    load().oracle_fc2d_local_solve(d(W), D, d(np.ascontiguousarray(q, dtype=np.float64)), mu,
+                                   d(P))
+    return P
+
+
+def fc2d_mutation(m):
+    load().oracle_fc2d_set_mutation(m)

@@ -25,6 +25,19 @@ class SparseBlockStructuredMatrix(C.Structure):
                 ("diagonal_blocks", c_size_t_p), ("version", C.c_uint64)]
 
 
+class CSparseMatrix(C.Structure):
+    _fields_ = [("nzmax", C.c_int64), ("m", C.c_int64), ("n", C.c_int64), ("p", c_int64_p),
+                ("i", c_int64_p), ("x", c_double_p), ("nz", C.c_int64)]
+
+
+class NumericsSparseMatrix(C.Structure):
+    _fields_ = [("triplet", C.POINTER(CSparseMatrix)), ("half_triplet", C.POINTER(CSparseMatrix)),
+                ("csc", C.POINTER(CSparseMatrix)), ("trans_csc", C.POINTER(CSparseMatrix)),
+                ("csr", C.POINTER(CSparseMatrix)), ("diag_indx", c_int64_p),
+                ("origin", C.c_int), ("linearSolverParams", C.c_void_p),
+                ("versions", C.c_uint64 * 5)]
+
+
 class NumericsMatrix(C.Structure):
     pass
 
@@ -32,7 +45,8 @@ class NumericsMatrix(C.Structure):
 NumericsMatrix._fields_ = [("storageType", C.c_int), ("size0", C.c_int), ("size1", C.c_int),
                            ("matrix0", c_double_p),
                            ("matrix1", C.POINTER(SparseBlockStructuredMatrix)),
-                           ("matrix2", C.c_void_p), ("internalData", C.c_void_p),
+                           ("matrix2", C.POINTER(NumericsSparseMatrix)),
+                           ("internalData", C.c_void_p),
                            ("version", C.c_uint64), ("destructible", C.POINTER(NumericsMatrix))]
 
 
@@ -91,6 +105,15 @@ class HipStats(C.Structure):
                 ("sweep_kernel_bytes_fp32", C.c_double)]
 
 
+class Fc2dParams(C.Structure):
+    _fields_ = [("max_iter", C.c_int), ("tol", C.c_double), ("error_evaluation", C.c_int)]
+
+
+class Fc2dStats(C.Structure):
+    _fields_ = [("info", C.c_int), ("iterations", C.c_int), ("residual", C.c_double),
+                ("num_colours", C.c_int), ("velocity_set", C.c_int), ("solve_ms", C.c_double)]
+
+
 class HipBatchStats(C.Structure):
     _fields_ = [("solve_ms", C.c_double), ("converged", C.c_int),
                 ("converged_contacts", C.c_int64), ("total_sweeps", C.c_int64),
@@ -146,6 +169,29 @@ SIGNATURES = {
     "frictionContact_new_from_filename": (FCP_p, [C.c_char_p]),
     "fc3d_checkTrivialCase": (C.c_int, [FCP_p, c_double_p, c_double_p, SO_p]),
     "fc3d_driver": (C.c_int, [FCP_p, c_double_p, c_double_p, SO_p]),
+    "fc2d_driver": (C.c_int, [FCP_p, c_double_p, c_double_p, SO_p]),
+    "fc2d_compute_error": (C.c_int, [FCP_p, c_double_p, c_double_p, C.c_double, C.c_double,
+                                     c_double_p]),
+    "NM_csc": (C.POINTER(CSparseMatrix), [C.POINTER(NumericsMatrix)]),
+    "NSM_new_from_file": (C.POINTER(NumericsSparseMatrix), [C.c_void_p]),
+    "NSM_write_in_file": (C.c_int, [C.POINTER(NumericsSparseMatrix), C.c_void_p]),
+    "NSM_free": (C.POINTER(NumericsSparseMatrix), [C.POINTER(NumericsSparseMatrix)]),
+    "fc2d_nsgs_hip": (None, [FCP_p, c_double_p, c_double_p, c_int_p, SO_p]),
+    "fc2d_nsgs_hip_set_default": (None, [SO_p]),
+    "fc2d_hip_params_default": (None, [C.c_void_p]),
+    "fc2d_problem_to_device": (C.c_int, [FCP_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "fc2d_hip_problem_create": (C.c_int, [C.c_int, c_int64_p, c_int64_p, c_double_p, C.c_int,
+                                          C.c_int, c_double_p, c_double_p, c_double_p, C.c_int,
+                                          C.POINTER(C.c_void_p)]),
+    "fc2d_hip_problem_destroy": (None, [C.c_void_p]),
+    "fc2d_hip_problem_num_colours": (C.c_int, [C.c_void_p]),
+    "fc2d_hip_problem_order": (C.c_int, [C.c_void_p, c_int_p]),
+    "fc2d_hip_set_reaction": (C.c_int, [C.c_void_p, c_double_p]),
+    "fc2d_hip_get_reaction": (C.c_int, [C.c_void_p, c_double_p]),
+    "fc2d_hip_get_velocity": (C.c_int, [C.c_void_p, c_double_p]),
+    "fc2d_hip_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fc2d_hip_compute_error": (C.c_int, [C.c_void_p, c_double_p, c_double_p, C.c_double,
+                                         c_double_p]),
     "fc3d_solver_options_create": (SO_p, [C.c_int]),
     "fc3d_compute_error": (C.c_int, [FCP_p, c_double_p, c_double_p, C.c_double, SO_p,
                                      C.c_double, c_double_p]),

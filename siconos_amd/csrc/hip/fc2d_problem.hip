@@ -1,0 +1,563 @@
+// fc2d_problem.hip — 2-D friction-contact NSGS on gfx950 (fc2d_hip.h).
+//
+// One workgroup of 1024 threads runs a whole solve of fc2d_nsgs
+// (fc2d_nsgs.c:297-538): the colours of the contact graph are swept one
+// after the other, separated by workgroup barriers, each thread taking the
+// colour's contacts in strides; the light error, the LIGHT_WITH_FULL_FINAL
+// tolerance logic and fc2d_compute_error (fc2d_compute_error.c:104-148) run
+// in the same kernel, so a solve is one launch with no host round trip.
+//
+// Data (colour-major slot order, one slot per contact):
+//   rows: 2 scalar rows per slot in CSR form, entries in the reference's
+//         summation order; column = slot-space scalar index 2*slot(d)+comp;
+//   diag: 2x2 diagonal block (column-major), det its determinant;
+//   q, mu, r, u: per slot.
+// Reductions (light error, norms, residual) sum each thread's slots in
+// order, then the threads in a fixed order: deterministic run to run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "fc2d_hip.h"
+#include "fc3d_internal.hpp"
+
+namespace fc3d {
+int colour_on_device(int nc, const int* d_rp, const int* d_ci, int* d_colour, hipStream_t s,
+                     int* ncolours_out, std::vector<int>* host_colour);
+}
+
+namespace {
+
+using fc3d::set_error;
+
+constexpr int FC2D_THREADS = 1024;
+
+struct Fc2dArgs {
+  int nc;
+  int ncolours;
+  const int* cptr;     // colour -> slot range
+  const int* rp;       // 2 nc + 1
+  const int* rc;       // entry -> slot-space scalar column
+  const double* rv;    // entry values
+  const double* diag;  // 4 per slot
+  const double* det;
+  const double* q;     // 2 per slot
+  const double* mu;
+  double* r;
+  double* u;
+  double* light;       // per slot, scratch
+  int row_mode;
+  int own_zero;
+  double norm_q;
+};
+
+struct Fc2dOut {
+  int iterations;
+  int notconv;
+  int velocity_set;
+  int pad;
+  double error;
+};
+
+// fc2d_nsgs.c:247-284
+__device__ __forceinline__ void local_solve(const double* W, double D, const double* q, double mu,
+                                            double* P) {
+  const double Wnn = W[0], Wtn = W[1], Wnt = W[2], Wtt = W[3];
+  if (q[0] > 0) {
+    P[0] = 0;
+    P[1] = 0;
+  } else {
+    P[0] = -(Wtt * q[0] - Wnt * q[1]) / D;
+    P[1] = -(-Wtn * q[0] + Wnn * q[1]) / D;
+    const double muPn = mu * P[0];
+    if (fabs(P[1]) > muPn) {
+      if (P[1] + muPn < 0) {
+        P[0] = -q[0] / (Wnn - mu * Wnt);
+        P[1] = -mu * P[0];
+      } else {
+        P[0] = -q[0] / (Wnn + mu * Wnt);
+        P[1] = mu * P[0];
+      }
+    }
+  }
+}
+
+// fc2d_compute_error.c:37-56
+__device__ __forceinline__ double unitary_error(const double* r, const double* u, double mu) {
+  double w0 = r[0] - u[0] - mu * fabs(u[1]);
+  double w1 = r[1] - u[1];
+  const double normT = fabs(w1);
+  if (mu * normT <= -w0) {
+    w0 = 0.;
+    w1 = 0.;
+  } else if (normT > mu * w0) {
+    w0 = (mu * normT + w0) / (mu * mu + 1.0);
+    w1 = mu * w0 * (w1 < 0 ? -1 : w1 > 0 ? 1 : 0);
+  }
+  w0 = r[0] - w0;
+  w1 = r[1] - w1;
+  return w0 * w0 + w1 * w1;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sum over the workgroup: waves by shuffles, then the 16 wave sums in order.
+__device__ double block_sum(double v, double* sh) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < FC2D_THREADS / 64; w++) t += sh[w];
+  return t;
+}
+
+// q_loc of slot s (fc2d_nsgs.c:97-113 through NM_row_prod_no_diag2)
+__device__ __forceinline__ void local_q(const Fc2dArgs& A, int s, double* y) {
+  for (int i = 0; i < 2; i++) {
+    const int row = 2 * s + i;
+    double acc = A.row_mode == FC2D_ROWS_DOT ? 0.0 : A.q[row];
+    for (int e = A.rp[row]; e < A.rp[row + 1]; e++) {
+      const int col = A.rc[e];
+      if ((col >> 1) == s) {
+        if (A.own_zero) acc += A.rv[e] * 0.0;
+      } else {
+        acc += A.rv[e] * A.r[col];
+      }
+    }
+    y[i] = A.row_mode == FC2D_ROWS_DOT ? A.q[row] + acc : acc;
+  }
+}
+
+// fc2d_compute_error with the resident r; writes u.  All threads return it.
+__device__ double full_error(const Fc2dArgs& A, double* sh) {
+  double e = 0.0, r2 = 0.0, u2 = 0.0;
+  for (int s = threadIdx.x; s < A.nc; s += FC2D_THREADS) {
+    double u[2];
+    for (int i = 0; i < 2; i++) {
+      const int row = 2 * s + i;
+      double acc = A.q[row];
+      for (int k = A.rp[row]; k < A.rp[row + 1]; k++) acc += A.rv[k] * A.r[A.rc[k]];
+      u[i] = acc;
+      A.u[row] = acc;
+    }
+    const double rr[2] = {A.r[2 * s], A.r[2 * s + 1]};
+    e += unitary_error(rr, u, A.mu[s]);
+    r2 += rr[0] * rr[0] + rr[1] * rr[1];
+    u2 += u[0] * u[0] + u[1] * u[1];
+  }
+  e = block_sum(e, sh);
+  r2 = block_sum(r2, sh);
+  u2 = block_sum(u2, sh);
+  double err = sqrt(e);
+  const double scale = fmax(sqrt(r2), fmax(sqrt(u2), A.norm_q));
+  if (fabs(scale) > DBL_EPSILON) err /= scale;
+  return err;
+}
+
+__global__ __launch_bounds__(FC2D_THREADS) void k_fc2d_error(Fc2dArgs A, double* out) {
+  __shared__ double sh[FC2D_THREADS / 64];
+  const double e = full_error(A, sh);
+  if (threadIdx.x == 0) *out = e;
+}
+
+// fc2d_nsgs.c:448-538 (the branch without freezing) in colour order
+__global__ __launch_bounds__(FC2D_THREADS) void k_fc2d_nsgs(Fc2dArgs A, int itermax, double tol,
+                                                           int mode, Fc2dOut* out) {
+  __shared__ double sh[FC2D_THREADS / 64];
+  int iter = 0, notconv = 1, vel = 0;
+  double error = INFINITY, tolerance = tol;
+  const bool light_mode = mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT ||
+                          mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT_WITH_FULL_FINAL;
+  while (iter < itermax && notconv) {
+    ++iter;
+    for (int c = 0; c < A.ncolours; c++) {
+      for (int s = A.cptr[c] + threadIdx.x; s < A.cptr[c + 1]; s += FC2D_THREADS) {
+        double ql[2], P[2] = {A.r[2 * s], A.r[2 * s + 1]};
+        local_q(A, s, ql);
+        local_solve(A.diag + 4 * s, A.det[s], ql, A.mu[s], P);
+        const double x0 = A.r[2 * s] - P[0], x1 = A.r[2 * s + 1] - P[1];
+        A.light[s] = x0 * x0 + x1 * x1;
+        A.r[2 * s] = P[0];
+        A.r[2 * s + 1] = P[1];
+      }
+      __syncthreads();  // the next colour reads this colour's reactions
+    }
+    if (light_mode) {
+      // fc2d_nsgs.c:136-142
+      double l = 0.0, r2 = 0.0;
+      for (int s = threadIdx.x; s < A.nc; s += FC2D_THREADS) {
+        l += A.light[s];
+        r2 += A.r[2 * s] * A.r[2 * s] + A.r[2 * s + 1] * A.r[2 * s + 1];
+      }
+      l = block_sum(l, sh);
+      r2 = block_sum(r2, sh);
+      error = sqrt(l);
+      const double norm_r = sqrt(r2);
+      if (fabs(norm_r) > DBL_EPSILON) error /= norm_r;
+      if (mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT) {
+        notconv = error < tolerance ? 0 : 1;
+      } else {
+        // fc2d_nsgs.c:191-228
+        notconv = 1;
+        if (error < tolerance) {
+          notconv = 0;
+          const double abs_err = full_error(A, sh);
+          vel = 1;
+          if (abs_err > tol) {
+            tolerance = error / abs_err * tol;
+            notconv = 1;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT_WITH_FULL_FINAL) {
+    error = full_error(A, sh);
+    vel = 1;
+    notconv = error < tol ? 0 : 1;
+  }
+  if (threadIdx.x == 0) {
+    out->iterations = iter;
+    out->notconv = notconv;
+    out->velocity_set = vel;
+    out->error = error;
+  }
+}
+
+template <typename T>
+int dev_upload(T** p, const std::vector<T>& h) {
+  FC3D_HIP_TRY(hipMalloc(p, sizeof(T) * (h.empty() ? 1 : h.size())));
+  if (!h.empty()) FC3D_HIP_TRY(hipMemcpy(*p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+}  // namespace
+
+struct fc2d_hip_problem {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nc = 0;
+  int ncolours = 0;
+  double norm_q = 0.0;
+  int row_mode = 0, own_zero = 0;
+  std::vector<int> order;  // slot -> contact
+  std::vector<int> slot;   // contact -> slot
+  int *cptr = nullptr, *rp = nullptr, *rc = nullptr;
+  double *rv = nullptr, *diag = nullptr, *det = nullptr, *q = nullptr, *mu = nullptr;
+  double *r = nullptr, *u = nullptr, *light = nullptr, *err = nullptr;
+  Fc2dOut* out = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+
+  Fc2dArgs args() const {
+    Fc2dArgs a;
+    a.nc = nc;
+    a.ncolours = ncolours;
+    a.cptr = cptr;
+    a.rp = rp;
+    a.rc = rc;
+    a.rv = rv;
+    a.diag = diag;
+    a.det = det;
+    a.q = q;
+    a.mu = mu;
+    a.r = r;
+    a.u = u;
+    a.light = light;
+    a.row_mode = row_mode;
+    a.own_zero = own_zero;
+    a.norm_q = norm_q;
+    return a;
+  }
+};
+
+extern "C" {
+
+void fc2d_hip_params_default(fc2d_hip_params* p) {
+  if (!p) return;
+  p->max_iter = 1000;
+  p->tol = 1e-4;
+  p->error_evaluation = SICONOS_NSGS_ERROR_EVALUATION_LIGHT_WITH_FULL_FINAL;
+}
+
+void fc2d_hip_problem_destroy(fc2d_hip_problem* h) {
+  if (!h) return;
+  if (fc3d::use_device(h->device) == 0) {
+    if (h->stream) hipStreamSynchronize(h->stream);
+    void* ptrs[] = {h->cptr, h->rp, h->rc, h->rv, h->diag, h->det, h->q,
+                    h->mu,   h->r,  h->u,  h->light, h->err, h->out};
+    for (void* p : ptrs)
+      if (p) hipFree(p);
+    for (hipEvent_t e : h->ev)
+      if (e) hipEventDestroy(e);
+    if (h->stream) hipStreamDestroy(h->stream);
+  }
+  delete h;
+}
+
+int fc2d_hip_problem_create(int nc, const int64_t* row_ptr, const int64_t* cols,
+                            const double* vals, int row_mode, int own_zero, const double* diag,
+                            const double* q, const double* mu, int device,
+                            fc2d_hip_problem** out) {
+  if (!out) return NUMERICS_ERR_NULL_POINTER;
+  *out = nullptr;
+  if (nc <= 0 || !row_ptr || !diag || !q || !mu ||
+      (row_mode != FC2D_ROWS_ACCUMULATE && row_mode != FC2D_ROWS_DOT)) {
+    set_error("fc2d_hip_problem_create: invalid arguments (nc=%d)", nc);
+    return NUMERICS_ERR_INVALID_ARGUMENT;
+  }
+  const int64_t n2 = 2 * (int64_t)nc, nnz = row_ptr[n2];
+  if (row_ptr[0] != 0 || nnz < 0 || nnz > INT32_MAX || (nnz > 0 && (!cols || !vals))) {
+    set_error("fc2d_hip_problem_create: bad row pointers");
+    return NUMERICS_ERR_INVALID_ARGUMENT;
+  }
+  for (int64_t i = 0; i < n2; i++)
+    if (row_ptr[i + 1] < row_ptr[i]) {
+      set_error("fc2d_hip_problem_create: row pointers decrease at row %lld", (long long)i);
+      return NUMERICS_ERR_INVALID_ARGUMENT;
+    }
+  for (int64_t e = 0; e < nnz; e++)
+    if (cols[e] < 0 || cols[e] >= n2) {
+      set_error("fc2d_hip_problem_create: column %lld out of range", (long long)cols[e]);
+      return NUMERICS_ERR_INVALID_ARGUMENT;
+    }
+  int rc = fc3d::use_device(device);
+  if (rc) return rc;
+  fc2d_hip_problem* h = new fc2d_hip_problem();
+  h->device = device;
+  h->nc = nc;
+  h->row_mode = row_mode;
+  h->own_zero = own_zero ? 1 : 0;
+  double nq = 0.0;
+  for (int64_t i = 0; i < n2; i++) nq += q[i] * q[i];
+  h->norm_q = sqrt(nq);
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&h->ev[0]) != hipSuccess || hipEventCreate(&h->ev[1]) != hipSuccess) {
+    set_error("fc2d_hip_problem_create: stream or event creation failed");
+    fc2d_hip_problem_destroy(h);
+    return NUMERICS_ERR_DEVICE;
+  }
+  // conflict graph: contacts referenced by a row, symmetrised, without self
+  std::vector<std::vector<int>> nb((size_t)nc);
+  for (int c = 0; c < nc; c++)
+    for (int i = 0; i < 2; i++)
+      for (int64_t e = row_ptr[2 * c + i]; e < row_ptr[2 * c + i + 1]; e++) {
+        const int d = (int)(cols[e] >> 1);
+        if (d != c) {
+          nb[(size_t)c].push_back(d);
+          nb[(size_t)d].push_back(c);
+        }
+      }
+  std::vector<int> grp((size_t)nc + 1, 0), gci;
+  for (int c = 0; c < nc; c++) {
+    auto& v = nb[(size_t)c];
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    grp[(size_t)c + 1] = grp[(size_t)c] + (int)v.size();
+  }
+  gci.reserve((size_t)grp[(size_t)nc]);
+  for (int c = 0; c < nc; c++) gci.insert(gci.end(), nb[(size_t)c].begin(), nb[(size_t)c].end());
+  nb.clear();
+  int *d_rp = nullptr, *d_ci = nullptr, *d_col = nullptr;
+  std::vector<int> colour;
+  if ((rc = dev_upload(&d_rp, grp)) || (rc = dev_upload(&d_ci, gci)) ||
+      hipMalloc(&d_col, sizeof(int) * (size_t)nc) != hipSuccess) {
+    if (!rc) set_error("fc2d_hip_problem_create: allocation failed");
+    hipFree(d_rp);
+    hipFree(d_ci);
+    fc2d_hip_problem_destroy(h);
+    return rc ? rc : NUMERICS_ERR_DEVICE;
+  }
+  rc = fc3d::colour_on_device(nc, d_rp, d_ci, d_col, h->stream, &h->ncolours, &colour);
+  hipFree(d_rp);
+  hipFree(d_ci);
+  hipFree(d_col);
+  if (rc) {
+    fc2d_hip_problem_destroy(h);
+    return rc;
+  }
+  // colour-major order, index order inside a colour
+  std::vector<int> cptr((size_t)h->ncolours + 1, 0);
+  for (int c = 0; c < nc; c++) cptr[(size_t)colour[(size_t)c] + 1]++;
+  for (int k = 0; k < h->ncolours; k++) cptr[(size_t)k + 1] += cptr[(size_t)k];
+  h->order.assign((size_t)nc, 0);
+  h->slot.assign((size_t)nc, 0);
+  {
+    std::vector<int> fill(cptr.begin(), cptr.end() - 1);
+    for (int c = 0; c < nc; c++) {
+      const int s = fill[(size_t)colour[(size_t)c]]++;
+      h->order[(size_t)s] = c;
+      h->slot[(size_t)c] = s;
+    }
+  }
+  // rows, diagonal blocks, q, mu in slot order
+  std::vector<int> srp((size_t)n2 + 1, 0), src((size_t)(nnz ? nnz : 0));
+  std::vector<double> srv((size_t)(nnz ? nnz : 0)), sdiag(4 * (size_t)nc), sdet((size_t)nc),
+      sq((size_t)n2), smu((size_t)nc);
+  for (int s = 0; s < nc; s++) {
+    const int c = h->order[(size_t)s];
+    for (int i = 0; i < 2; i++) {
+      const int64_t row = 2 * (int64_t)c + i;
+      srp[2 * (size_t)s + i + 1] = (int)(row_ptr[row + 1] - row_ptr[row]);
+    }
+  }
+  for (int64_t i = 0; i < n2; i++) srp[(size_t)i + 1] += srp[(size_t)i];
+  for (int s = 0; s < nc; s++) {
+    const int c = h->order[(size_t)s];
+    for (int i = 0; i < 2; i++) {
+      const int64_t row = 2 * (int64_t)c + i;
+      int k = srp[2 * (size_t)s + i];
+      for (int64_t e = row_ptr[row]; e < row_ptr[row + 1]; e++, k++) {
+        const int d = (int)(cols[e] >> 1), comp = (int)(cols[e] & 1);
+        src[(size_t)k] = 2 * h->slot[(size_t)d] + comp;
+        srv[(size_t)k] = vals[e];
+      }
+      sq[2 * (size_t)s + i] = q[row];
+    }
+    for (int e = 0; e < 4; e++) sdiag[4 * (size_t)s + e] = diag[4 * (size_t)c + e];
+    // fc2d_nsgs.c:229-233
+    const double* B = diag + 4 * (size_t)c;
+    sdet[(size_t)s] = B[0] * B[3] - B[1] * B[2];
+    smu[(size_t)s] = mu[c];
+  }
+  std::vector<double> zero((size_t)n2, 0.0);
+  if ((rc = dev_upload(&h->cptr, cptr)) || (rc = dev_upload(&h->rp, srp)) ||
+      (rc = dev_upload(&h->rc, src)) || (rc = dev_upload(&h->rv, srv)) ||
+      (rc = dev_upload(&h->diag, sdiag)) || (rc = dev_upload(&h->det, sdet)) ||
+      (rc = dev_upload(&h->q, sq)) || (rc = dev_upload(&h->mu, smu)) ||
+      (rc = dev_upload(&h->r, zero)) || (rc = dev_upload(&h->u, zero)) ||
+      (rc = dev_upload(&h->light, std::vector<double>((size_t)nc, 0.0)))) {
+    fc2d_hip_problem_destroy(h);
+    return rc;
+  }
+  if (hipMalloc(&h->err, sizeof(double)) != hipSuccess ||
+      hipMalloc(&h->out, sizeof(Fc2dOut)) != hipSuccess) {
+    set_error("fc2d_hip_problem_create: allocation failed");
+    fc2d_hip_problem_destroy(h);
+    return NUMERICS_ERR_DEVICE;
+  }
+  *out = h;
+  return 0;
+}
+
+int fc2d_hip_problem_num_colours(const fc2d_hip_problem* h) { return h ? h->ncolours : -1; }
+
+int fc2d_hip_problem_order(const fc2d_hip_problem* h, int* order) {
+  if (!h || !order) return NUMERICS_ERR_NULL_POINTER;
+  std::memcpy(order, h->order.data(), sizeof(int) * (size_t)h->nc);
+  return 0;
+}
+
+static int copy_in(fc2d_hip_problem* h, const double* host, double* dev) {
+  std::vector<double> s(2 * (size_t)h->nc);
+  for (int k = 0; k < h->nc; k++) {
+    const int c = h->order[(size_t)k];
+    s[2 * (size_t)k] = host[2 * (size_t)c];
+    s[2 * (size_t)k + 1] = host[2 * (size_t)c + 1];
+  }
+  FC3D_HIP_TRY(hipMemcpyAsync(dev, s.data(), sizeof(double) * s.size(), hipMemcpyHostToDevice,
+                              h->stream));
+  FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+static int copy_out(fc2d_hip_problem* h, const double* dev, double* host) {
+  std::vector<double> s(2 * (size_t)h->nc);
+  FC3D_HIP_TRY(hipMemcpyAsync(s.data(), dev, sizeof(double) * s.size(), hipMemcpyDeviceToHost,
+                              h->stream));
+  FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
+  for (int k = 0; k < h->nc; k++) {
+    const int c = h->order[(size_t)k];
+    host[2 * (size_t)c] = s[2 * (size_t)k];
+    host[2 * (size_t)c + 1] = s[2 * (size_t)k + 1];
+  }
+  return 0;
+}
+
+int fc2d_hip_set_reaction(fc2d_hip_problem* h, const double* r) {
+  if (!h || !r) return NUMERICS_ERR_NULL_POINTER;
+  int rc = fc3d::use_device(h->device);
+  return rc ? rc : copy_in(h, r, h->r);
+}
+
+int fc2d_hip_get_reaction(fc2d_hip_problem* h, double* r) {
+  if (!h || !r) return NUMERICS_ERR_NULL_POINTER;
+  int rc = fc3d::use_device(h->device);
+  return rc ? rc : copy_out(h, h->r, r);
+}
+
+int fc2d_hip_get_velocity(fc2d_hip_problem* h, double* u) {
+  if (!h || !u) return NUMERICS_ERR_NULL_POINTER;
+  int rc = fc3d::use_device(h->device);
+  return rc ? rc : copy_out(h, h->u, u);
+}
+
+int fc2d_hip_solve(fc2d_hip_problem* h, const fc2d_hip_params* p, fc2d_hip_stats* s) {
+  if (!h || !p || !s) return NUMERICS_ERR_NULL_POINTER;
+  if (p->max_iter < 0 || !(p->tol >= 0.0)) {
+    set_error("fc2d_hip_solve: max_iter %d / tol %g out of range", p->max_iter, p->tol);
+    return NUMERICS_ERR_INVALID_ARGUMENT;
+  }
+  int rc = fc3d::use_device(h->device);
+  if (rc) return rc;
+  std::memset(s, 0, sizeof *s);
+  s->num_colours = h->ncolours;
+  // fc2d_nsgs.c:228-245, 318-334: a diagonal block with det < DBL_EPSILON
+  // stops the solve before any sweep
+  {
+    std::vector<double> det((size_t)h->nc);
+    FC3D_HIP_TRY(hipMemcpy(det.data(), h->det, sizeof(double) * det.size(),
+                           hipMemcpyDeviceToHost));
+    for (double d : det)
+      if (d < DBL_EPSILON) {
+        s->info = 1;
+        s->iterations = 0;
+        s->residual = INFINITY;
+        return 0;
+      }
+  }
+  FC3D_HIP_TRY(hipEventRecord(h->ev[0], h->stream));
+  hipLaunchKernelGGL(k_fc2d_nsgs, dim3(1), dim3(FC2D_THREADS), 0, h->stream, h->args(),
+                     p->max_iter, p->tol, p->error_evaluation, h->out);
+  FC3D_HIP_TRY(hipGetLastError());
+  FC3D_HIP_TRY(hipEventRecord(h->ev[1], h->stream));
+  Fc2dOut o;
+  FC3D_HIP_TRY(hipMemcpyAsync(&o, h->out, sizeof o, hipMemcpyDeviceToHost, h->stream));
+  FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
+  s->info = o.notconv;
+  s->iterations = o.iterations;
+  s->residual = o.error;
+  s->velocity_set = o.velocity_set;
+  s->solve_ms = ms;
+  return 0;
+}
+
+int fc2d_hip_compute_error(fc2d_hip_problem* h, const double* reaction, double* velocity,
+                           double norm, double* error) {
+  if (!h || !error) return NUMERICS_ERR_NULL_POINTER;
+  int rc = fc3d::use_device(h->device);
+  if (rc) return rc;
+  if (reaction && (rc = copy_in(h, reaction, h->r))) return rc;
+  Fc2dArgs a = h->args();
+  a.norm_q = norm;
+  hipLaunchKernelGGL(k_fc2d_error, dim3(1), dim3(FC2D_THREADS), 0, h->stream, a, h->err);
+  FC3D_HIP_TRY(hipGetLastError());
+  FC3D_HIP_TRY(hipMemcpyAsync(error, h->err, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
+  if (velocity && (rc = copy_out(h, h->u, velocity))) return rc;
+  return 0;
+}
+
+}  // extern "C"

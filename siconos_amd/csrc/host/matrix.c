@@ -114,6 +114,8 @@ NumericsMatrix* NM_create_from_data(int storageType, int size0, int size1, void*
     m->matrix0 = (double*)data;
   else if (storageType == NM_SPARSE_BLOCK)
     m->matrix1 = (SparseBlockStructuredMatrix*)data;
+  else if (storageType == NM_SPARSE)
+    m->matrix2 = (NumericsSparseMatrix*)data;
   m->destructible = m;
   return m;
 }
@@ -122,6 +124,7 @@ NumericsMatrix* NM_free(NumericsMatrix* m) {
   if (!m) return NULL;
   free(m->matrix0);
   SBM_free_all(m->matrix1);
+  NSM_free(m->matrix2);
   free(m);
   return NULL;
 }

@@ -2,8 +2,8 @@
  * Format and behaviour of numerics/src/FrictionContact/FrictionContactProblem.c:67-200,
  * numerics/src/linalg/NumericsMatrix.c:1551-1700 (NM_write_in_file /
  * NM_new_from_file, dense branch) and SparseBlockMatrix.c:1619-1800
- * (SBM_write_in_file / SBM_new_from_file).  Compressed (NM_SPARSE) storage is
- * refused with NUMERICS_ERR_NOT_IMPLEMENTED semantics (NULL). */
+ * (SBM_write_in_file / SBM_new_from_file), for dimension 2 or 3.  Compressed
+ * (NM_SPARSE) storage is read and written by sparse.c. */
 #include <errno.h>
 #include <stdlib.h>
 #include <string.h>
@@ -33,7 +33,7 @@ void frictionContactProblem_free(FrictionContactProblem* p) {
   free(p);
 }
 
-static void sbm_write(const SparseBlockStructuredMatrix* m, FILE* f) {
+static void sbm_write(const SparseBlockStructuredMatrix* m, FILE* f, int dim) {
   fprintf(f, "%zu\n", m->nbblocks);
   if (m->nbblocks == 0) return;
   fprintf(f, "%zu\n%zu\n", m->blocknumber0, m->blocknumber1);
@@ -48,7 +48,7 @@ static void sbm_write(const SparseBlockStructuredMatrix* m, FILE* f) {
   for (size_t row = 0; row + 1 < m->filled1; row++)
     for (size_t k = m->index1_data[row]; k < m->index1_data[row + 1]; k++) {
       fprintf(f, "%zu\n", k);
-      for (int e = 0; e < 9; e++) fprintf(f, "%32.24e\n", m->block[k][e]);
+      for (int e = 0; e < dim * dim; e++) fprintf(f, "%32.24e\n", m->block[k][e]);
     }
 }
 
@@ -64,7 +64,10 @@ int frictionContact_printInFile(FrictionContactProblem* p, FILE* f) {
       if ((i + 1) % m->size1 == 0) fprintf(f, "\n");
     }
   } else if (m->storageType == NM_SPARSE_BLOCK) {
-    sbm_write(m->matrix1, f);
+    sbm_write(m->matrix1, f, p->dimension);
+  } else if (m->storageType == NM_SPARSE && m->matrix2) {
+    int rc = NSM_write_in_file(m->matrix2, f);
+    if (rc) return rc;
   } else {
     return NUMERICS_ERR_NOT_IMPLEMENTED;
   }
@@ -83,7 +86,7 @@ int frictionContact_printInFilename(FrictionContactProblem* p, char* filename) {
   return rc;
 }
 
-static SparseBlockStructuredMatrix* sbm_read(FILE* f) {
+static SparseBlockStructuredMatrix* sbm_read(FILE* f, int dim) {
   SparseBlockStructuredMatrix* m = SBM_new();
   size_t v;
   if (fscanf(f, "%zu", &m->nbblocks) != 1 || m->nbblocks == 0) goto fail;
@@ -91,10 +94,12 @@ static SparseBlockStructuredMatrix* sbm_read(FILE* f) {
   m->blocksize0 = (size_t*)malloc(sizeof(size_t) * m->blocknumber0);
   m->blocksize1 = (size_t*)malloc(sizeof(size_t) * m->blocknumber1);
   for (size_t i = 0; i < m->blocknumber0; i++) {
-    if (fscanf(f, "%zu", &m->blocksize0[i]) != 1 || m->blocksize0[i] != 3 * (i + 1)) goto fail;
+    if (fscanf(f, "%zu", &m->blocksize0[i]) != 1 || m->blocksize0[i] != (size_t)dim * (i + 1))
+      goto fail;
   }
   for (size_t i = 0; i < m->blocknumber1; i++) {
-    if (fscanf(f, "%zu", &m->blocksize1[i]) != 1 || m->blocksize1[i] != 3 * (i + 1)) goto fail;
+    if (fscanf(f, "%zu", &m->blocksize1[i]) != 1 || m->blocksize1[i] != (size_t)dim * (i + 1))
+      goto fail;
   }
   if (fscanf(f, "%zu", &m->filled1) != 1 || fscanf(f, "%zu", &m->filled2) != 1) goto fail;
   m->index1_data = (size_t*)malloc(sizeof(size_t) * m->filled1);
@@ -107,8 +112,8 @@ static SparseBlockStructuredMatrix* sbm_read(FILE* f) {
   for (size_t row = 0; row + 1 < m->filled1; row++)
     for (size_t k = m->index1_data[row]; k < m->index1_data[row + 1]; k++) {
       if (k >= m->nbblocks || fscanf(f, "%zu", &v) != 1) goto fail;
-      m->block[k] = (double*)malloc(9 * sizeof(double));
-      for (int e = 0; e < 9; e++)
+      m->block[k] = (double*)malloc((size_t)(dim * dim) * sizeof(double));
+      for (int e = 0; e < dim * dim; e++)
         if (fscanf(f, "%lf", &m->block[k][e]) != 1) goto fail;
     }
   return m;
@@ -123,7 +128,7 @@ FrictionContactProblem* frictionContact_newFromFile(FILE* f) {
   if (fscanf(f, "%d", &d) != 1 || fscanf(f, "%d", &nc) != 1) return NULL;
   if (fscanf(f, "%d", &st) != 1 || fscanf(f, "%d", &s0) != 1 || fscanf(f, "%d", &s1) != 1)
     return NULL;
-  if (d != 3 || nc <= 0 || s0 != 3 * nc || s1 != 3 * nc) return NULL;
+  if ((d != 2 && d != 3) || nc <= 0 || s0 != d * nc || s1 != d * nc) return NULL;
   NumericsMatrix* M = NULL;
   if (st == NM_DENSE) {
     int a, b;
@@ -136,11 +141,15 @@ FrictionContactProblem* frictionContact_newFromFile(FILE* f) {
       }
     M = NM_create_from_data(NM_DENSE, s0, s1, data);
   } else if (st == NM_SPARSE_BLOCK) {
-    SparseBlockStructuredMatrix* S = sbm_read(f);
+    SparseBlockStructuredMatrix* S = sbm_read(f, d);
     if (!S) return NULL;
     M = NM_create_from_data(NM_SPARSE_BLOCK, s0, s1, S);
+  } else if (st == NM_SPARSE) {
+    NumericsSparseMatrix* S = NSM_new_from_file(f);
+    if (!S) return NULL;
+    M = NM_create_from_data(NM_SPARSE, s0, s1, S);
   } else {
-    return NULL; /* compressed storage: not on this path */
+    return NULL;
   }
   double* q = (double*)malloc(sizeof(double) * (size_t)s1);
   double* mu = (double*)malloc(sizeof(double) * (size_t)nc);
@@ -148,7 +157,7 @@ FrictionContactProblem* frictionContact_newFromFile(FILE* f) {
     if (fscanf(f, "%lf", &q[i]) != 1) goto fail;
   for (int i = 0; i < nc; i++)
     if (fscanf(f, "%lf", &mu[i]) != 1) goto fail;
-  return frictionContactProblem_new_with_data(3, (size_t)nc, M, q, mu);
+  return frictionContactProblem_new_with_data(d, (size_t)nc, M, q, mu);
 fail:
   free(q);
   free(mu);

@@ -29,6 +29,11 @@ SICONOS_FRICTION_3D_NSGS_HIP_PRECISION = 20
 SICONOS_FRICTION_3D_NSGS_HIP_WARM_START = 21
 SICONOS_FRICTION_3D_NSGS_HIP_SWEEPS_PER_LAUNCH = 23
 SICONOS_FRICTION_3D_NSGS_HIP_NUM_COLOURS = 25
+SICONOS_NSGS_FREEZING_CONTACT = 15
+SICONOS_FRICTION_2D_NSGS_HIP = 440
+NUMERICS_ERR_INVALID_ARGUMENT = -2
+NUMERICS_ERR_INVALID_SOLVER = -4
+NUMERICS_ERR_NOT_IMPLEMENTED = -5
 NUMERICS_ERR_MAX_ITER = -20
 
 _libc = C.CDLL(None)
@@ -98,10 +103,11 @@ class SparseBlock3:
 
 
 class FrictionContactProblem:
-    """A 3-D friction-contact problem (FrictionContactProblem.h:33-46).
+    """A friction-contact problem (FrictionContactProblem.h:33-46).
 
-    ``M`` is a SparseBlock3 or a dense (3nc x 3nc) array.  The C struct is
-    built on demand and owned by this object."""
+    Built from Python for dimension 3: ``M`` is a SparseBlock3 or a dense
+    (3nc x 3nc) array.  ``from_file`` also reads dimension 2 and compressed
+    (NM_SPARSE) matrices.  The C struct is owned by this object."""
 
     def __init__(self, dim=3, M=None, q=None, mu=None, _cptr=None):
         self._lib = capi.load()
@@ -141,9 +147,43 @@ class FrictionContactProblem:
         return self._p.contents.numberOfContacts
 
     @property
+    def dimension(self):
+        return self._p.contents.dimension
+
+    @property
+    def storage(self):
+        return self._p.contents.M.contents.storageType
+
+    @property
     def q(self):
-        n = 3 * self.numberOfContacts
+        n = self.dimension * self.numberOfContacts
         return np.ctypeslib.as_array(self._p.contents.q, shape=(n,)).copy()
+
+    def dense(self):
+        """W as a dense array, whatever its storage (duplicates summed)."""
+        M = self._p.contents.M.contents
+        n0, n1 = M.size0, M.size1
+        if M.storageType == 0:
+            D = np.ctypeslib.as_array(M.matrix0, shape=(n0 * n1,))
+            return D.reshape(n0, n1, order="F").copy()
+        out = np.zeros((n0, n1))
+        if M.storageType == 1:
+            S = M.matrix1.contents
+            d = self.dimension
+            for r in range(S.filled1 - 1):
+                for k in range(S.index1_data[r], S.index1_data[r + 1]):
+                    c = S.index2_data[k]
+                    b = np.ctypeslib.as_array(S.block[k], shape=(d * d,)).reshape(d, d, order="F")
+                    out[d * r:d * r + d, d * c:d * c + d] += b
+            return out
+        A = self._lib.NM_csc(self._p.contents.M).contents
+        nnz = A.p[A.n]
+        p = np.ctypeslib.as_array(A.p, shape=(A.n + 1,))
+        i = np.ctypeslib.as_array(A.i, shape=(max(nnz, 1),))[:nnz]
+        x = np.ctypeslib.as_array(A.x, shape=(max(nnz, 1),))[:nnz]
+        for j in range(A.n):
+            np.add.at(out[:, j], i[p[j]:p[j + 1]], x[p[j]:p[j + 1]])
+        return out
 
     @property
     def mu(self):
@@ -214,6 +254,84 @@ def fc3d_driver(problem, reaction, velocity, options):
     """numerics fc3d_driver (fc3d_driver.c:77-145); reaction is in/out."""
     assert reaction.dtype == np.float64 and velocity.dtype == np.float64
     return capi.load().fc3d_driver(problem._p, dptr(reaction), dptr(velocity), options._p)
+
+
+def fc2d_driver(problem, reaction, velocity, options):
+    """numerics fc2d_driver (fc2d_driver.c:38-150); reaction is in/out."""
+    assert reaction.dtype == np.float64 and velocity.dtype == np.float64
+    return capi.load().fc2d_driver(problem._p, dptr(reaction), dptr(velocity), options._p)
+
+
+def fc2d_compute_error(problem, z, w, tolerance, norm):
+    """fc2d_compute_error (fc2d_compute_error.c:104-148), on the GPU;
+    returns (info, error)."""
+    err = C.c_double(0.0)
+    info = capi.load().fc2d_compute_error(problem._p, dptr(z), dptr(w), tolerance, norm,
+                                           C.byref(err))
+    return info, err.value
+
+
+class Fc2dHipProblem:
+    """Resident fc2d problem of the C-ABI (fc2d_hip.h), uploaded from a
+    dimension-2 FrictionContactProblem."""
+
+    def __init__(self, problem, device=0):
+        self._lib = capi.load()
+        self.nc = problem.numberOfContacts
+        h = C.c_void_p()
+        rc = self._lib.fc2d_problem_to_device(problem._p, device, C.byref(h))
+        if rc:
+            raise RuntimeError(f"fc2d_problem_to_device: {rc} {capi.last_error()}")
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.fc2d_hip_problem_destroy(h)
+            self._h = None
+
+    @property
+    def num_colours(self):
+        return self._lib.fc2d_hip_problem_num_colours(self._h)
+
+    def order(self):
+        o = np.zeros(self.nc, dtype=np.int32)
+        self._lib.fc2d_hip_problem_order(self._h, iptr(o))
+        return o
+
+    def set_reaction(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        self._check(self._lib.fc2d_hip_set_reaction(self._h, dptr(r)), "fc2d_hip_set_reaction")
+
+    def reaction(self):
+        r = np.zeros(2 * self.nc)
+        self._check(self._lib.fc2d_hip_get_reaction(self._h, dptr(r)), "fc2d_hip_get_reaction")
+        return r
+
+    def velocity(self):
+        u = np.zeros(2 * self.nc)
+        self._check(self._lib.fc2d_hip_get_velocity(self._h, dptr(u)), "fc2d_hip_get_velocity")
+        return u
+
+    def solve(self, max_iter=1000, tol=1e-4, error_evaluation=2):
+        p = capi.Fc2dParams(max_iter, tol, error_evaluation)
+        s = capi.Fc2dStats()
+        self._check(self._lib.fc2d_hip_solve(self._h, C.byref(p), C.byref(s)), "fc2d_hip_solve")
+        return s
+
+    def compute_error(self, r=None, norm=None):
+        e = C.c_double()
+        u = np.zeros(2 * self.nc)
+        rr = None if r is None else np.ascontiguousarray(r, dtype=np.float64)
+        self._check(self._lib.fc2d_hip_compute_error(self._h, dptr(rr) if rr is not None else None,
+                                                      dptr(u), 0.0 if norm is None else norm,
+                                                      C.byref(e)), "fc2d_hip_compute_error")
+        return e.value, u
+
+    @staticmethod
+    def _check(rc, what):
+        if rc:
+            raise RuntimeError(f"{what}: {rc} {capi.last_error()}")
 
 
 def fc3d_compute_error(problem, z, w, tolerance, options, norm):

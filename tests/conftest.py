@@ -33,6 +33,36 @@ def fixture_path(name):
     return os.path.join(FIXTURES, name)
 
 
+_UNZIPPED = {}
+
+
+def fixture_file(name):
+    """Path of a fixture; a `.gz` copy is decompressed once into a temporary
+    directory."""
+    plain = os.path.join(FIXTURES, name)
+    if os.path.exists(plain):
+        return plain
+    if name not in _UNZIPPED:
+        import gzip
+        import tempfile
+        out = os.path.join(tempfile.mkdtemp(prefix="fixtures_"), name)
+        with gzip.open(plain + ".gz", "rb") as f, open(out, "wb") as g:
+            g.write(f.read())
+        _UNZIPPED[name] = out
+    return _UNZIPPED[name]
+
+
+FC2D_COLLECTION = [
+    # data_collection_fc2d_1.c, in its order
+    "FrictionContactProblem00374.dat", "FrictionContactProblem00394.dat",
+    "FC2D_SliderCrankLagrangian00000.dat", "FC2D_SliderCrankLagrangian00001.dat",
+    "FrictionContactProblem00727.dat", "FrictionContactProblem00031.dat",
+    "FrictionContactProblem00071.dat", "FrictionContactProblem00237.dat",
+    "fc2d_granularflowonwall_00667_00000.dat", "fc2d_granularflowonwall_02020_00213.dat",
+    "fc2d_granularflowonwall_07002_00801.dat",
+]
+
+
 def read_ac_inputs():
     """ACinputs.dat (reference test data): n, then 3n reactions, 3n
     velocities, n mu, 3n rho (fc3d_AlartCurnierFunctions_test.c:34-70)."""

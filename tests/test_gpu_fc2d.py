@@ -1,0 +1,124 @@
+"""fc2d NSGS on the GPU (fc2d_driver with SICONOS_FRICTION_2D_NSGS_HIP, and
+the fc2d_hip.h device layer) against the oracle.
+
+The reference's own test (test_fc2d_1.c on data_collection_fc2d_1.c: tol
+1e-5, 10000 iterations) is run through the drop-in driver; its CUDA
+counterpart (test_fc2d_gpu.c) runs the same collection.  The GPU sweeps in
+colour order; the oracle run in that order gives the same reactions bit for
+bit and the same iteration count.
+"""
+import numpy as np
+import pytest
+
+from conftest import FC2D_COLLECTION, fixture_file
+from oracle import pyoracle as O
+from siconos_amd import numerics as N
+
+pytestmark = pytest.mark.gpu
+
+FC2D_NSGS_HIP = 440
+
+
+def _options(tol=1e-5, max_iter=10000):
+    o = N.SolverOptions(FC2D_NSGS_HIP)
+    o.dparam[N.SICONOS_DPARAM_TOL] = tol
+    o.iparam[N.SICONOS_IPARAM_MAX_ITER] = max_iter
+    return o
+
+
+@pytest.mark.parametrize("name", FC2D_COLLECTION)
+def test_driver_matches_the_oracle_in_colour_order(lib, name):
+    path = fixture_file(name)
+    p = N.FrictionContactProblem.from_file(path)
+    nc = p.numberOfContacts
+    r = np.zeros(2 * nc)
+    u = np.zeros(2 * nc)
+    o = _options()
+    info = N.fc2d_driver(p, r, u, o)
+    it = int(o.iparam[N.SICONOS_IPARAM_ITER_DONE])
+    res = float(o.dparam[N.SICONOS_DPARAM_RESIDU])
+    order = N.Fc2dHipProblem(p).order()
+    info_o, r_o, u_o, it_o, res_o = O.Fc2dProblem(path).nsgs(tol=1e-5, max_iter=10000, mode=2,
+                                                             order=order)
+    print(f"\n[fc2d] {name}: {nc} contacts, {it} sweeps, residual {res:.3e}, "
+          f"{int(o.iparam[25])} colours, {o.dparam[24]:.2f} ms")
+    assert info == info_o == 0
+    assert it == it_o
+    np.testing.assert_array_equal(r, r_o)
+    np.testing.assert_allclose(u, u_o, rtol=1e-13, atol=1e-13 * np.abs(u_o).max())
+    assert abs(res - res_o) <= 1e-12 * max(res_o, 1e-300) + 1e-300
+    assert res <= 1e-5
+
+
+def test_light_mode_max_iter_and_warm_start(lib):
+    path = fixture_file("FrictionContactProblem00237.dat")
+    p = N.FrictionContactProblem.from_file(path)
+    H = N.Fc2dHipProblem(p)
+    order = H.order()
+    orc = O.Fc2dProblem(path)
+    # LIGHT: stops on the light error, no velocity
+    s = H.solve(max_iter=10000, tol=1e-5, error_evaluation=1)
+    info_o, r_o, _, it_o, res_o = orc.nsgs(tol=1e-5, max_iter=10000, mode=1, order=order)
+    assert s.info == info_o == 0 and s.iterations == it_o and not s.velocity_set
+    np.testing.assert_array_equal(H.reaction(), r_o)
+    # FULL is not evaluated by fc2d_nsgs: max_iter sweeps, info 1, residual inf
+    H.set_reaction(np.zeros(2 * p.numberOfContacts))
+    s = H.solve(max_iter=7, tol=1e-5, error_evaluation=0)
+    assert s.info == 1 and s.iterations == 7 and s.residual == np.inf
+    # max_iter bounds the sweeps: info 1
+    H.set_reaction(np.zeros(2 * p.numberOfContacts))
+    s = H.solve(max_iter=3, tol=1e-12, error_evaluation=2)
+    info_o, r_o, u_o, it_o, res_o = orc.nsgs(tol=1e-12, max_iter=3, mode=2, order=order)
+    assert s.info == info_o == 1 and s.iterations == 3
+    np.testing.assert_array_equal(H.reaction(), r_o)
+    # the resident reaction is the start of the next solve
+    s = H.solve(max_iter=10000, tol=1e-5)
+    info_o, r_o, _, it_o, _ = orc.nsgs(tol=1e-5, max_iter=10000, mode=2, order=order,
+                                       reaction=r_o)
+    assert s.info == 0 and s.iterations == it_o
+    np.testing.assert_array_equal(H.reaction(), r_o)
+
+
+@pytest.mark.parametrize("name", ["FrictionContactProblem00374.dat",
+                                  "FrictionContactProblem00727.dat",
+                                  "fc2d_granularflowonwall_02020_00213.dat"])
+def test_compute_error_matches_the_oracle(lib, name):
+    path = fixture_file(name)
+    p = N.FrictionContactProblem.from_file(path)
+    orc = O.Fc2dProblem(path)
+    rng = np.random.default_rng(5)
+    z = rng.normal(size=2 * p.numberOfContacts)
+    w = np.zeros_like(z)
+    e_o, w_o = orc.compute_error(z)
+    norm = float(np.sqrt(np.sum(p.q * p.q)))
+    info, e = N.fc2d_compute_error(p, z, w, 1e-5, norm)
+    np.testing.assert_array_equal(w, w_o)
+    assert abs(e - e_o) <= 1e-13 * e_o
+    assert info == (1 if e > 1e-5 else 0)
+
+
+def test_refusals(lib, tmp_path):
+    p2 = N.FrictionContactProblem.from_file(fixture_file("FrictionContactProblem00031.dat"))
+    r = np.zeros(2 * p2.numberOfContacts)
+    u = np.zeros_like(r)
+    o = _options()
+    o.iparam[N.SICONOS_NSGS_FREEZING_CONTACT] = 20
+    assert N.fc2d_driver(p2, r, u, o) == N.NUMERICS_ERR_NOT_IMPLEMENTED
+    o3 = N.SolverOptions(540)  # an fc3d solver is not an fc2d solver
+    assert N.fc2d_driver(p2, r, u, o3) == N.NUMERICS_ERR_INVALID_SOLVER
+    p3 = N.FrictionContactProblem.from_file(fixture_file("FC3D_Example1_SBM.dat"))
+    r3 = np.zeros(9)
+    assert N.fc2d_driver(p3, r3, r3.copy(), _options()) == N.NUMERICS_ERR_INVALID_ARGUMENT
+    # a singular diagonal block: info 1 before any sweep (fc2d_nsgs.c:318-334)
+    path = tmp_path / "singular.dat"
+    W = np.diag([1.0, 1.0, 0.0, 1.0])
+    txt = "2\n2\n0\n4\n4\n4\t4\n" + "\n".join(" ".join(f"{v:.17g}" for v in W[:, j])
+                                             for j in range(4))
+    txt += "\n-1 0 -1 0\n0.5 0.5\n"
+    path.write_text(txt)
+    ps = N.FrictionContactProblem.from_file(str(path))
+    r = np.zeros(4)
+    o = _options()
+    assert N.fc2d_driver(ps, r, np.zeros(4), o) == 1
+    assert o.iparam[N.SICONOS_IPARAM_ITER_DONE] == 0
+    assert o.dparam[N.SICONOS_DPARAM_RESIDU] == np.inf

@@ -14,7 +14,7 @@ from siconos_amd import numerics as N
 
 def declared_symbols():
     names = set()
-    for h in ("fc3d_numerics.h", "fc3d_hip.h"):
+    for h in sorted(os.listdir(os.path.join(REPO, "include"))):
         text = open(os.path.join(REPO, "include", h)).read()
         for m in re.finditer(r"^FC3D_API\s+[^;(]*?\**\s*(\w+)\s*\(", text, re.M):
             names.add(m.group(1))
@@ -191,3 +191,28 @@ def test_sbm_from_dense_keeps_nonzero_blocks(lib):
     s = S.contents
     assert s.nbblocks == 5  # 3 diagonal (kept even if zero) + 2 coupling
     lib.SBM_free_all(S)
+
+
+def test_fc2d_row_builder_keeps_the_heap_intact():
+    """Regression: fc2d_driver builds the scalar rows of every storage on the
+    host before any device call (an SBM block puts four entries in its two
+    rows).  Run in a child process with glibc's heap checking on; without a
+    GPU the solve itself returns NUMERICS_ERR_DEVICE."""
+    import subprocess
+    import sys
+    from conftest import FC2D_COLLECTION, fixture_file
+    paths = [fixture_file(n) for n in FC2D_COLLECTION]
+    code = (
+        "import sys, numpy as np\n"
+        f"sys.path.insert(0, {REPO!r})\n"
+        "from siconos_amd import numerics as N\n"
+        f"for path in {paths!r}:\n"
+        "    p = N.FrictionContactProblem.from_file(path)\n"
+        "    r = np.zeros(2 * p.numberOfContacts)\n"
+        "    rc = N.fc2d_driver(p, r, r.copy(), N.SolverOptions(440))\n"
+        "    assert rc in (0, -60), rc\n"
+        "print('ok')\n")
+    env = dict(os.environ, MALLOC_CHECK_="3")
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0 and out.stdout.strip().endswith("ok"), out.stderr[-2000:]
