@@ -154,6 +154,54 @@ def batch_leg(steps, warmup, cpu):
     return out
 
 
+def fc2d_leg(steps, warmup, cpu):
+    """The reference's largest fc2d fixture (fc2d_granularflowonwall_07002_00801,
+    7002 contacts, CSparse storage, from data_collection_fc2d_1.c) solved by
+    the fc2d HIP path at tol 1e-5 from a zero reaction, resident on the
+    device; the oracle's fc2d_nsgs (natural order, one core) beside it."""
+    import gzip
+    import tempfile
+    from siconos_amd import numerics as N
+    src = os.path.join(REPO, "tests", "golden", "fixtures",
+                       "fc2d_granularflowonwall_07002_00801.dat.gz")
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "fc2d.dat")
+        with gzip.open(src, "rb") as f, open(path, "wb") as g:
+            g.write(f.read())
+        p = N.FrictionContactProblem.from_file(path)
+        H = N.Fc2dHipProblem(p)
+        nc = p.numberOfContacts
+        zero = np.zeros(2 * nc)
+        for _ in range(warmup):
+            H.set_reaction(zero)
+            H.solve(max_iter=10000, tol=1e-5)
+        total, dev = 0.0, 0.0
+        for _ in range(steps):
+            H.set_reaction(zero)
+            t0 = time.perf_counter()
+            s = H.solve(max_iter=10000, tol=1e-5)
+            total += time.perf_counter() - t0
+            dev += s.solve_ms
+        out = {"workload": "fc2d_granularflowonwall_07002_00801 (7002 contacts, CSparse), "
+                           "tol 1e-5", "contacts": nc, "colours": H.num_colours,
+               "sweeps": s.iterations, "residual": s.residual, "info": s.info,
+               "ms_per_step": total / steps * 1e3, "device_ms": dev / steps,
+               "value": nc * steps / total if s.info == 0 else 0.0, "unit": "contact-solves/s"}
+        log(f"[bench] fc2d: {out['ms_per_step']:.1f} ms, {s.iterations} sweeps, "
+            f"residual {s.residual:.2e}")
+        if cpu:
+            from oracle import pyoracle as O
+            o = O.Fc2dProblem(path)
+            t0 = time.perf_counter()
+            info, r, u, it, res = o.nsgs(tol=1e-5, max_iter=10000, mode=2)
+            dt = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": nc / dt if info == 0 else 0.0,
+                                   "unit": "contact-solves/s", "cores": 1, "kind": "port",
+                                   "sample": f"the whole solve, oracle fc2d_nsgs in natural "
+                                             f"order ({dt * 1e3:.0f} ms, {it} sweeps)"}
+    return out
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of known size: its per-dispatch
     FETCH_SIZE and WRITE_SIZE readings convert the counters to bytes."""
@@ -266,6 +314,7 @@ def main():
         cpu = cpu_baseline(g, sp.iterations, args.cpu_sweeps)
 
     batch = None if args.no_batch else batch_leg(args.steps, args.warmup, not args.no_cpu)
+    fc2d = fc2d_leg(args.steps, args.warmup, not args.no_cpu)
 
     # W = H M^-1 H^T and q = H v_free for the same pile, assembled on the GPU
     asm_ms = []
@@ -318,6 +367,7 @@ def main():
         "warm_start_training": {"contacts": train_nc, "final_loss": mlp_loss},
         "speedup_vs_plain": (value / plain_value) if plain_value else None,
         "roofline": roofline, "cpu_baseline": cpu, "batch": batch, "assembly": assembly,
+        "fc2d": fc2d,
     }
     print(json.dumps(out))
 

@@ -121,20 +121,49 @@ __device__ double block_sum(double v, double* sh) {
   return t;
 }
 
+// Entries of a row are loaded CHUNK at a time (columns and values, then the
+// reactions they select) before being added in order, so the loads of a
+// chunk overlap instead of forming one dependent chain per entry.
+constexpr int CHUNK = 8;
+
+// acc += the row's terms in order; own-contact entries (column in the pair
+// of slot s) add w * 0 when own_zero, else nothing.  x: the reactions read
+// (r for the sweep's local problem; own entries never read r there).
+template <bool RESIDUAL>
+__device__ __forceinline__ double row_terms(const Fc2dArgs& A, int s, int beg, int end,
+                                            double acc) {
+  for (int e0 = beg; e0 < end; e0 += CHUNK) {
+    int col[CHUNK];
+    double w[CHUNK], x[CHUNK];
+#pragma unroll
+    for (int k = 0; k < CHUNK; k++)
+      if (e0 + k < end) {
+        col[k] = A.rc[e0 + k];
+        w[k] = A.rv[e0 + k];
+      }
+#pragma unroll
+    for (int k = 0; k < CHUNK; k++)
+      if (e0 + k < end) x[k] = (RESIDUAL || (col[k] >> 1) != s) ? A.r[col[k]] : 0.0;
+#pragma unroll
+    for (int k = 0; k < CHUNK; k++)
+      if (e0 + k < end) {
+        if (RESIDUAL || (col[k] >> 1) != s)
+          acc += w[k] * x[k];
+        else if (A.own_zero)
+          acc += w[k] * 0.0;
+      }
+  }
+  return acc;
+}
+
 // q_loc of slot s (fc2d_nsgs.c:97-113 through NM_row_prod_no_diag2)
 __device__ __forceinline__ void local_q(const Fc2dArgs& A, int s, double* y) {
   for (int i = 0; i < 2; i++) {
     const int row = 2 * s + i;
-    double acc = A.row_mode == FC2D_ROWS_DOT ? 0.0 : A.q[row];
-    for (int e = A.rp[row]; e < A.rp[row + 1]; e++) {
-      const int col = A.rc[e];
-      if ((col >> 1) == s) {
-        if (A.own_zero) acc += A.rv[e] * 0.0;
-      } else {
-        acc += A.rv[e] * A.r[col];
-      }
-    }
-    y[i] = A.row_mode == FC2D_ROWS_DOT ? A.q[row] + acc : acc;
+    if (A.row_mode == FC2D_ROWS_DOT)
+      y[i] = A.q[row] + row_terms<false>(A, s, A.rp[row], A.rp[row + 1], 0.0);
+    else
+      y[i] = row_terms<false>(A, s, A.rp[row], A.rp[row + 1], A.q[row]);
   }
 }
 
@@ -145,10 +174,8 @@ __device__ double full_error(const Fc2dArgs& A, double* sh) {
     double u[2];
     for (int i = 0; i < 2; i++) {
       const int row = 2 * s + i;
-      double acc = A.q[row];
-      for (int k = A.rp[row]; k < A.rp[row + 1]; k++) acc += A.rv[k] * A.r[A.rc[k]];
-      u[i] = acc;
-      A.u[row] = acc;
+      u[i] = row_terms<true>(A, s, A.rp[row], A.rp[row + 1], A.q[row]);
+      A.u[row] = u[i];
     }
     const double rr[2] = {A.r[2 * s], A.r[2 * s + 1]};
     e += unitary_error(rr, u, A.mu[s]);
