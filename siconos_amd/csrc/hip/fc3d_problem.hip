@@ -529,6 +529,7 @@ struct fc3d_hip_problem {
   hipStream_t stream = nullptr;
   std::vector<int> colour;            // per original contact
   std::vector<int> colour_slice_ptr;  // ncolours + 1
+  int* d_cps = nullptr;               // the same on the device
   std::vector<int> slot2orig;         // -1 for padding
   std::vector<int> orig2slot;
   std::vector<double> colour_bytes64, colour_bytes32;  // algorithmic bytes per colour pass
@@ -570,7 +571,7 @@ struct fc3d_hip_problem {
   int64_t* d_block_dst = nullptr;
   int* d_o2s = nullptr;
   // velocity form (fc3d_hip_problem_set_bodies)
-  int nbodies = 0, minv_diag = 1, vel_on = 0, graph_vel[2] = {0, 0};
+  int nbodies = 0, minv_diag = 1, vel_on = 0, graph_vel[2] = {0, 0}, graph_tail[2] = {0, 0};
   int *d_ba = nullptr, *d_bb = nullptr, *d_inc_ptr = nullptr, *d_inc = nullptr;
   double *d_H = nullptr, *d_minv = nullptr, *d_vfree = nullptr;
   double4* d_V = nullptr;
@@ -736,6 +737,48 @@ static void launch_sweep_vel(fc3d_hip_problem* h, int phase, int sb, int se, hip
   }
 }
 
+// First colour of the velocity-form tail (k_sweep_vel_tail): the run of last
+// colours with at most two rounds of the tail block's waves each; ncolours
+// if that run is shorter than two colours, or with FC3D_HIP_VEL_TAIL=0 in
+// the environment (one launch per colour; the results are the same).
+static int vel_tail_begin(const fc3d_hip_problem* h, int phase) {
+  const char* env = std::getenv("FC3D_HIP_VEL_TAIL");
+  if (env && env[0] == '0') return h->ncolours;
+  const int waves = (phase ? vel_tail_threads<float>() : vel_tail_threads<double>()) / 64;
+  int c = h->ncolours;
+  while (c > 0 && h->colour_slice_ptr[(size_t)c] - h->colour_slice_ptr[(size_t)c - 1] <= 2 * waves)
+    c--;
+  return h->ncolours - c >= 2 ? c : h->ncolours;
+}
+
+static void launch_sweep_vel_tail(fc3d_hip_problem* h, int phase, int c0, hipEvent_t e0,
+                                  hipEvent_t e1) {
+  const dim3 g(1);
+  if (phase) {
+    auto k = k_sweep_vel_tail<float>;
+    const dim3 b(vel_tail_threads<float>());
+    SweepPtrs<float> P = ptrs32(h);
+    if (h->accel_on) P.f = h->aa32.f;
+    if (e0)
+      hipExtLaunchKernelGGL(k, g, b, 0, h->stream, e0, e1, 0, P, bodies32(h), h->prm,
+                            (const SolveState*)h->st, (const int*)h->d_cps, c0, h->ncolours, 1);
+    else
+      hipLaunchKernelGGL(k, g, b, 0, h->stream, P, bodies32(h), h->prm, h->st, h->d_cps, c0,
+                         h->ncolours, 1);
+  } else {
+    auto k = k_sweep_vel_tail<double>;
+    const dim3 b(vel_tail_threads<double>());
+    SweepPtrs<double> P = ptrs64(h, h->part);
+    if (h->accel_on) P.f = h->aa64.f;
+    if (e0)
+      hipExtLaunchKernelGGL(k, g, b, 0, h->stream, e0, e1, 0, P, bodies64(h), h->prm,
+                            (const SolveState*)h->st, (const int*)h->d_cps, c0, h->ncolours, 0);
+    else
+      hipLaunchKernelGGL(k, g, b, 0, h->stream, P, bodies64(h), h->prm, h->st, h->d_cps, c0,
+                         h->ncolours, 0);
+  }
+}
+
 // Body velocities from the phase's reaction (st / as as in k_vinit).
 static void launch_vinit(fc3d_hip_problem* h, int phase, const SolveState* st,
                          const AccelState* as) {
@@ -785,11 +828,22 @@ static void launch_sweep_df(fc3d_hip_problem* h, int phase, hipEvent_t e0, hipEv
 }
 
 // Event pairs one timed sweep uses, and the algorithmic bytes of pair c.
-static int timed_launches(const fc3d_hip_problem* h, int schedule) {
-  return schedule == FC3D_SCHEDULE_DATAFLOW && !h->vel_on ? 1 : h->ncolours;
+static int vel_tail_begin(const fc3d_hip_problem* h, int phase);
+static int timed_launches(const fc3d_hip_problem* h, int phase, int schedule) {
+  if (h->vel_on) {
+    const int ct = vel_tail_begin(h, phase);
+    return ct + (ct < h->ncolours ? 1 : 0);
+  }
+  return schedule == FC3D_SCHEDULE_DATAFLOW ? 1 : h->ncolours;
 }
 static double timed_bytes(const fc3d_hip_problem* h, int phase, int schedule, int c) {
-  if (h->vel_on) return phase ? h->vel_bytes32[(size_t)c] : h->vel_bytes64[(size_t)c];
+  if (h->vel_on) {
+    const std::vector<double>& b = phase ? h->vel_bytes32 : h->vel_bytes64;
+    if (c < vel_tail_begin(h, phase)) return b[(size_t)c];
+    double t = 0.0;  // the tail launch: all its colours
+    for (size_t k = (size_t)c; k < b.size(); k++) t += b[k];
+    return t;
+  }
   const std::vector<double>& b = phase ? h->colour_bytes32 : h->colour_bytes64;
   if (schedule != FC3D_SCHEDULE_DATAFLOW) return b[(size_t)c];
   double t = 0.0;
@@ -800,10 +854,15 @@ static double timed_bytes(const fc3d_hip_problem* h, int phase, int schedule, in
 static void enqueue_sweep(fc3d_hip_problem* h, int phase, int schedule, hipEvent_t* ev,
                           int nev) {
   if (h->vel_on) {
-    for (int c = 0; c < h->ncolours; c++) {
+    const int ct = vel_tail_begin(h, phase);
+    for (int c = 0; c < ct; c++) {
       const int sb = h->colour_slice_ptr[(size_t)c], se = h->colour_slice_ptr[(size_t)c + 1];
       const bool t = ev && 2 * c + 1 < nev;
       launch_sweep_vel(h, phase, sb, se, t ? ev[2 * c] : nullptr, t ? ev[2 * c + 1] : nullptr);
+    }
+    if (ct < h->ncolours) {
+      const bool t = ev && 2 * ct + 1 < nev;
+      launch_sweep_vel_tail(h, phase, ct, t ? ev[2 * ct] : nullptr, t ? ev[2 * ct + 1] : nullptr);
     }
   } else if (schedule == FC3D_SCHEDULE_DATAFLOW) {
     launch_sweep_df(h, phase, ev && nev >= 2 ? ev[0] : nullptr, ev && nev >= 2 ? ev[1] : nullptr);
@@ -1009,7 +1068,8 @@ static int reset_accel(fc3d_hip_problem* h) {
 
 static int build_graph(fc3d_hip_problem* h, int phase, int sweeps, int schedule) {
   if (h->graph[phase] && h->graph_sweeps[phase] == sweeps && h->graph_schedule[phase] == schedule &&
-      h->graph_accel[phase] == h->accel_on && h->graph_vel[phase] == h->vel_on)
+      h->graph_accel[phase] == h->accel_on && h->graph_vel[phase] == h->vel_on &&
+      h->graph_tail[phase] == vel_tail_begin(h, phase))
     return 0;
   if (h->graph[phase]) {
     hipGraphExecDestroy(h->graph[phase]);
@@ -1025,6 +1085,7 @@ static int build_graph(fc3d_hip_problem* h, int phase, int sweeps, int schedule)
   h->graph_schedule[phase] = schedule;
   h->graph_accel[phase] = h->accel_on;
   h->graph_vel[phase] = h->vel_on;
+  h->graph_tail[phase] = vel_tail_begin(h, phase);
   return 0;
 }
 
@@ -1167,7 +1228,7 @@ static int tune_schedule(fc3d_hip_problem* h, int phase, double norm_q, double l
       FC3D_HIP_TRY(hipGetLastError());
       FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
       if (rep == 0) continue;  // first sweep warms caches and code
-      for (int c = 0; c < timed_launches(h, sc); c++) {
+      for (int c = 0; c < timed_launches(h, phase, sc); c++) {
         float ms = 0.f;
         hipEventElapsedTime(&ms, ev[2 * (size_t)c], ev[2 * (size_t)c + 1]);
         ms_total += ms;
@@ -1293,6 +1354,7 @@ int fc3d_hip_problem_create(int nc, const int64_t* row_ptr, const int64_t* col_i
   h->nslots = L.nslots;
   h->nplanes = L.nplanes;
   h->colour_slice_ptr = L.colour_slice_ptr;
+  if ((rc = dev_upload(&h->d_cps, h->colour_slice_ptr))) return fail(rc);
   h->slot2orig = L.slot2orig;
   h->orig2slot = L.orig2slot;
   h->colour_bytes64 = L.colour_bytes64;
@@ -1332,7 +1394,7 @@ void fc3d_hip_problem_destroy(fc3d_hip_problem* h) {
     if (p) hipFree(p);
   free_accel(h);
   void* dfp[] = {h->dep_ptr, h->dep_idx, h->qlist, h->done,    h->df,    h->d_block_dst,
-                 h->d_o2s,    h->d_ba,     h->d_bb,   h->d_inc_ptr, h->d_inc, h->d_H,
+                 h->d_o2s,    h->d_cps,    h->d_ba,   h->d_bb,   h->d_inc_ptr, h->d_inc, h->d_H,
                  h->d_minv,   h->d_vfree,  h->d_V,    h->d_H32,     h->d_minv32, h->d_V32};
   for (void* p : dfp)
     if (p) hipFree(p);
@@ -1847,7 +1909,7 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
         FC3D_HIP_TRY(hipGetLastError());
         FC3D_HIP_TRY(hipMemcpyAsync(&cur, h->st, sizeof cur, hipMemcpyDeviceToHost, h->stream));
         FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
-        for (int c = 0; c < timed_launches(h, sched[phase]); c++) {
+        for (int c = 0; c < timed_launches(h, phase, sched[phase]); c++) {
           float ms = 0.f;
           hipEventElapsedTime(&ms, kev[2 * (size_t)c], kev[2 * (size_t)c + 1]);
           const double by = timed_bytes(h, phase, sched[phase], c);

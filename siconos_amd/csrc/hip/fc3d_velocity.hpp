@@ -81,23 +81,22 @@ __global__ void k_vinit(BodyPtrs<T> B, const typename vec4<T>::type* __restrict_
   B.V[2 * (size_t)k + 1] = v1;
 }
 
-// One colour of one sweep in the velocity form.
-template <typename T>
-__global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void k_sweep_vel(SweepPtrs<T> P,
-                                                                          BodyPtrs<T> B,
-                                                      const DevParams* __restrict__ prm,
-                                                      const SolveState* __restrict__ st,
-                                                      int s_begin, int s_end, int want_phase) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int s = __builtin_amdgcn_readfirstlane(s_begin + xcd_block() * 4 + wave);
-  if (s >= s_end) return;
+// One slice (64 contacts) of one colour in the velocity form, for the
+// calling wave; writes the slice's partial sums.  With CHECK, the stop flag
+// is tested after the slot's loads are issued (the caller tests it
+// otherwise) and the function returns without writing if it is set.
+template <typename T, bool CHECK>
+__device__ __forceinline__ void vel_slice(const SweepPtrs<T>& P, const BodyPtrs<T>& B,
+                                          const DevParams* __restrict__ prm,
+                                          const SolveState* __restrict__ st, int s, int lane,
+                                          int want_phase) {
   const int slot = s * 64 + lane;
   double light = 0.0, nr2 = 0.0;
   // Every load of the slot is issued before the stop flag is tested, so the
   // flag's latency overlaps theirs (all addresses are in bounds: padding
   // slots read body 0).  The empty asm makes the loaded values live before
   // the test, which keeps the compiler from sinking the loads below it.
-  const int stop = st->stop, phase = st->phase;
+  const int stop = CHECK ? st->stop : 0, phase = CHECK ? st->phase : want_phase;
   const int len = P.len[slot];
   SlotInputs<T> in;
   load_slot<T>(P, s, lane, &in);
@@ -112,9 +111,11 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void k_sweep_vel(Sweep
     va[side][0] = B.V[2 * k];
     va[side][1] = B.V[2 * k + 1];
   }
-  asm volatile("" ::"v"(len), "v"(in.lp.W[0]), "v"(in.lp.mu), "v"(in.ro.x), "v"(in.pli),
-               "v"(h[0]), "v"(h[35]), "v"(va[0][0].x), "v"(va[1][1].x), "s"(stop), "s"(phase));
-  if (stop || phase != want_phase) return;
+  if (CHECK) {
+    asm volatile("" ::"v"(len), "v"(in.lp.W[0]), "v"(in.lp.mu), "v"(in.ro.x), "v"(in.pli),
+                 "v"(h[0]), "v"(h[35]), "v"(va[0][0].x), "v"(va[1][1].x), "s"(stop), "s"(phase));
+    if (stop || phase != want_phase) return;
+  }
   if (len >= 0) {
     const T tol = T(want_phase ? prm->fp32_local_tol : st->local_tol);
     // u = H_a v_a + H_b v_b, then q_loc = u - W_cc r_old
@@ -159,6 +160,43 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void k_sweep_vel(Sweep
   if (lane == 0) {
     P.part[2 * (size_t)s] = light;
     P.part[2 * (size_t)s + 1] = nr2;
+  }
+}
+
+// One colour of one sweep in the velocity form: one wave per slice.
+template <typename T>
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void k_sweep_vel(SweepPtrs<T> P,
+                                                                          BodyPtrs<T> B,
+                                                      const DevParams* __restrict__ prm,
+                                                      const SolveState* __restrict__ st,
+                                                      int s_begin, int s_end, int want_phase) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s = __builtin_amdgcn_readfirstlane(s_begin + xcd_block() * 4 + wave);
+  if (s >= s_end) return;
+  vel_slice<T, true>(P, B, prm, st, s, lane, want_phase);
+}
+
+// The last colours of a sweep, each of at most two rounds of the block's
+// waves, in one workgroup: colour after colour, separated by workgroup
+// barriers, in place of one launch per colour.  The same per-slice
+// computation and partial sums as k_sweep_vel, so the result is identical.
+template <typename T>
+constexpr int vel_tail_threads() {
+  return sizeof(T) == 8 ? 256 : 512;  // the waves k_sweep_vel's registers allow per CU
+}
+
+template <typename T>
+__global__ __launch_bounds__(vel_tail_threads<T>()) void k_sweep_vel_tail(
+    SweepPtrs<T> P, BodyPtrs<T> B, const DevParams* __restrict__ prm,
+    const SolveState* __restrict__ st, const int* __restrict__ colour_slices, int c_begin,
+    int c_end, int want_phase) {
+  if (st->stop || st->phase != want_phase) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  constexpr int waves = vel_tail_threads<T>() / 64;
+  for (int c = c_begin; c < c_end; c++) {
+    const int sb = colour_slices[c], se = colour_slices[c + 1];
+    for (int s = sb + wave; s < se; s += waves) vel_slice<T, false>(P, B, prm, st, s, lane, want_phase);
+    __syncthreads();  // the next colour reads this colour's body velocities
   }
 }
 

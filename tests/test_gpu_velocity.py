@@ -144,3 +144,28 @@ def test_driver_maps_the_fp32_local_solver(lib):
     o.dparam[26] = 123.0
     r[:] = 0.0
     assert N.fc3d_driver(prob, r, u, o) != 0
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_tail_launch_gives_the_same_iterates(lib, precision):
+    """The last small colours run in one workgroup (k_sweep_vel_tail) instead
+    of one launch each: same per-slice computation, so the same iterates and
+    sweep counts bit for bit (FC3D_HIP_VEL_TAIL=0 turns the tail off)."""
+    import os
+    g = N.generate_granular_pile(20000, 50000, 0.1, 0.9, 5)
+    H = _with_bodies(g)
+    prm = N.hip_params(tol=1e-6, max_iter=3000, warm_start=1, velocity_form=1,
+                       precision=precision, fp32_tol=1e-6, fp32_local_solver=550)
+    a = H.solve(prm)
+    ra = H.reaction()
+    os.environ["FC3D_HIP_VEL_TAIL"] = "0"
+    try:
+        b = H.solve(prm)
+    finally:
+        del os.environ["FC3D_HIP_VEL_TAIL"]
+    assert a.iterations == b.iterations and a.info == b.info == 0
+    np.testing.assert_array_equal(H.reaction(), ra)
+    # instrumented mode times the tail as one launch
+    i = H.solve(N.hip_params(tol=1e-300, max_iter=5, warm_start=1, velocity_form=1,
+                             precision=precision, fp32_tol=1e-300, instrument=1))
+    assert i.sweep_kernel_launches < 5 * H.num_colours
