@@ -178,6 +178,7 @@ SIGNATURES = {
     "NSM_free": (C.POINTER(NumericsSparseMatrix), [C.POINTER(NumericsSparseMatrix)]),
     "fc2d_nsgs_hip": (None, [FCP_p, c_double_p, c_double_p, c_int_p, SO_p]),
     "fc2d_nsgs_hip_set_default": (None, [SO_p]),
+    "fc3d_hip_problem_body_layout": (C.c_int, [C.c_void_p]),
     "fc2d_hip_params_default": (None, [C.c_void_p]),
     "fc2d_problem_to_device": (C.c_int, [FCP_p, C.c_int, C.POINTER(C.c_void_p)]),
     "fc2d_hip_problem_create": (C.c_int, [C.c_int, c_int64_p, c_int64_p, c_double_p, C.c_int,

@@ -574,6 +574,9 @@ struct fc3d_hip_problem {
   int nbodies = 0, minv_diag = 1, vel_on = 0, graph_vel[2] = {0, 0}, graph_tail[2] = {0, 0};
   int *d_ba = nullptr, *d_bb = nullptr, *d_inc_ptr = nullptr, *d_inc = nullptr;
   double *d_H = nullptr, *d_minv = nullptr, *d_vfree = nullptr;
+  double* d_F = nullptr;  // frames layout (rigid-body blocks), instead of d_H
+  float* d_F32 = nullptr;
+  int frames = 0;
   double4* d_V = nullptr;
   float *d_H32 = nullptr, *d_minv32 = nullptr;
   float4* d_V32 = nullptr;
@@ -612,13 +615,14 @@ static SweepPtrs<float> ptrs32(fc3d_hip_problem* h) {
 
 static BodyPtrs<double> bodies64(fc3d_hip_problem* h) {
   BodyPtrs<double> B;
-  B.ba = h->d_ba; B.bb = h->d_bb; B.H = h->d_H; B.minv = h->d_minv; B.minv_diag = h->minv_diag;
+  B.ba = h->d_ba; B.bb = h->d_bb; B.H = h->d_H; B.F = h->d_F; B.minv = h->d_minv;
+  B.minv_diag = h->minv_diag;
   B.V = h->d_V; B.vfree = h->d_vfree; B.inc_ptr = h->d_inc_ptr; B.inc = h->d_inc;
   return B;
 }
 static BodyPtrs<float> bodies32(fc3d_hip_problem* h) {
   BodyPtrs<float> B;
-  B.ba = h->d_ba; B.bb = h->d_bb; B.H = h->d_H32; B.minv = h->d_minv32;
+  B.ba = h->d_ba; B.bb = h->d_bb; B.H = h->d_H32; B.F = h->d_F32; B.minv = h->d_minv32;
   B.minv_diag = h->minv_diag; B.V = h->d_V32; B.vfree = h->d_vfree; B.inc_ptr = h->d_inc_ptr;
   B.inc = h->d_inc;
   return B;
@@ -717,7 +721,7 @@ static void launch_sweep_vel(fc3d_hip_problem* h, int phase, int sb, int se, hip
                              hipEvent_t e1) {
   const dim3 g((unsigned)((se - sb + 3) / 4)), b(256);
   if (phase) {
-    auto k = k_sweep_vel<float>;
+    auto k = h->frames ? k_sweep_vel<float, true> : k_sweep_vel<float, false>;
     SweepPtrs<float> P = ptrs32(h);
     if (h->accel_on) P.f = h->aa32.f;
     if (e0)
@@ -726,7 +730,7 @@ static void launch_sweep_vel(fc3d_hip_problem* h, int phase, int sb, int se, hip
     else
       hipLaunchKernelGGL(k, g, b, 0, h->stream, P, bodies32(h), h->prm, h->st, sb, se, 1);
   } else {
-    auto k = k_sweep_vel<double>;
+    auto k = h->frames ? k_sweep_vel<double, true> : k_sweep_vel<double, false>;
     SweepPtrs<double> P = ptrs64(h, h->part);
     if (h->accel_on) P.f = h->aa64.f;
     if (e0)
@@ -755,7 +759,7 @@ static void launch_sweep_vel_tail(fc3d_hip_problem* h, int phase, int c0, hipEve
                                   hipEvent_t e1) {
   const dim3 g(1);
   if (phase) {
-    auto k = k_sweep_vel_tail<float>;
+    auto k = h->frames ? k_sweep_vel_tail<float, true> : k_sweep_vel_tail<float, false>;
     const dim3 b(vel_tail_threads<float>());
     SweepPtrs<float> P = ptrs32(h);
     if (h->accel_on) P.f = h->aa32.f;
@@ -766,7 +770,7 @@ static void launch_sweep_vel_tail(fc3d_hip_problem* h, int phase, int c0, hipEve
       hipLaunchKernelGGL(k, g, b, 0, h->stream, P, bodies32(h), h->prm, h->st, h->d_cps, c0,
                          h->ncolours, 1);
   } else {
-    auto k = k_sweep_vel_tail<double>;
+    auto k = h->frames ? k_sweep_vel_tail<double, true> : k_sweep_vel_tail<double, false>;
     const dim3 b(vel_tail_threads<double>());
     SweepPtrs<double> P = ptrs64(h, h->part);
     if (h->accel_on) P.f = h->aa64.f;
@@ -784,10 +788,12 @@ static void launch_vinit(fc3d_hip_problem* h, int phase, const SolveState* st,
                          const AccelState* as) {
   const dim3 g((unsigned)((h->nbodies + 255) / 256)), b(256);
   if (phase)
-    hipLaunchKernelGGL(k_vinit<float>, g, b, 0, h->stream, bodies32(h), (const float4*)h->r32,
+    hipLaunchKernelGGL(h->frames ? (k_vinit<float, true>) : (k_vinit<float, false>), g, b, 0,
+                       h->stream, bodies32(h), (const float4*)h->r32,
                        h->nbodies, st, as, 1);
   else
-    hipLaunchKernelGGL(k_vinit<double>, g, b, 0, h->stream, bodies64(h), (const double4*)h->r,
+    hipLaunchKernelGGL(h->frames ? (k_vinit<double, true>) : (k_vinit<double, false>), g, b, 0,
+                       h->stream, bodies64(h), (const double4*)h->r,
                        h->nbodies, st, as, 0);
 }
 
@@ -961,15 +967,16 @@ static int ensure_accel(fc3d_hip_problem* h, int mixed) {
 
 // FP32 copies of the velocity-form factors.
 static int ensure_vel32(fc3d_hip_problem* h) {
-  if (h->d_H32) return 0;
-  const size_t nh = (size_t)h->nslices * 36 * 64;
+  if (h->d_H32 || h->d_F32) return 0;
+  const size_t nh = (size_t)h->nslices * (h->frames ? 15 : 36) * 64;
   const size_t nm = (size_t)h->nbodies * (h->minv_diag ? 6 : 36);
   int rc;
-  if ((rc = dev_alloc(&h->d_H32, nh)) || (rc = dev_alloc(&h->d_minv32, nm)) ||
+  float** dst = h->frames ? &h->d_F32 : &h->d_H32;
+  if ((rc = dev_alloc(dst, nh)) || (rc = dev_alloc(&h->d_minv32, nm)) ||
       (rc = dev_alloc(&h->d_V32, 2 * (size_t)h->nbodies)))
     return rc;
   hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((nh + 255) / 256)), dim3(256), 0,
-                     h->stream, h->d_H, h->d_H32, nh);
+                     h->stream, h->frames ? h->d_F : h->d_H, *dst, nh);
   hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((nm + 255) / 256)), dim3(256), 0,
                      h->stream, h->d_minv, h->d_minv32, nm);
   FC3D_HIP_TRY(hipGetLastError());
@@ -988,7 +995,7 @@ static void free_accel_bodies(fc3d_hip_problem* h) {
 // Frees the attached factors of W and everything sized by the bodies.
 static void drop_bodies(fc3d_hip_problem* h) {
   void* old[] = {h->d_ba, h->d_bb, h->d_inc_ptr, h->d_inc, h->d_H, h->d_minv, h->d_vfree, h->d_V,
-                 h->d_H32, h->d_minv32, h->d_V32};
+                 h->d_H32, h->d_minv32, h->d_V32, h->d_F, h->d_F32};
   for (void* q : old)
     if (q) hipFree(q);
   h->d_ba = h->d_bb = h->d_inc_ptr = h->d_inc = nullptr;
@@ -996,6 +1003,9 @@ static void drop_bodies(fc3d_hip_problem* h) {
   h->d_V = nullptr;
   h->d_H32 = h->d_minv32 = nullptr;
   h->d_V32 = nullptr;
+  h->d_F = nullptr;
+  h->d_F32 = nullptr;
+  h->frames = 0;
   h->nbodies = 0;
   free_accel_bodies(h);
   // captured velocity-form sweeps hold the freed pointers
@@ -1250,6 +1260,84 @@ static int tune_schedule(fc3d_hip_problem* h, int phase, double norm_q, double l
   return 0;
 }
 
+// Frames layout detection: every contact's blocks must be H_a = [T | T S(l_a)]
+// and H_b = -[T | T S(l_b)] for one 3x3 T and lever arms l_a, l_b (a side
+// without a body is free).  l is read off the skew matrix sg T^-1 Rot, and the
+// blocks rebuilt the way the kernels rebuild them (frames_to_h) must equal
+// the given ones to 1e-13 of the block's largest entry.  On success Fs holds
+// [slice][15][64]: T row-major, l_a, l_b.
+static bool rigid_frames(int nc, const int* body_a, const int* body_b, const double* H,
+                         const std::vector<int>& orig2slot, int nslices, std::vector<double>* Fs) {
+  Fs->assign((size_t)nslices * 15 * 64, 0.0);
+  for (int c = 0; c < nc; c++) {
+    const double* hc = H + 36 * (size_t)c;
+    const bool has[2] = {body_a[c] >= 0, body_b[c] >= 0};
+    double f[15] = {0};
+    // T from side a, or from side b's negated translational block
+    const int ref = has[0] ? 0 : has[1] ? 1 : -1;
+    if (ref >= 0) {
+      const double sg = ref ? -1.0 : 1.0;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) f[3 * i + j] = sg * hc[18 * ref + i + 3 * j];
+    }
+    double big = 0.0;
+    for (int e = 0; e < 36; e++) big = std::max(big, std::fabs(hc[e]));
+    // T^-1 by cofactors
+    const double* T = f;
+    const double det = T[0] * (T[4] * T[8] - T[5] * T[7]) - T[1] * (T[3] * T[8] - T[5] * T[6]) +
+                       T[2] * (T[3] * T[7] - T[4] * T[6]);
+    if (ref >= 0 && !(std::fabs(det) > 1e-12 * big * big * big)) return false;
+    double Ti[9] = {0};
+    if (ref >= 0) {
+      Ti[0] = (T[4] * T[8] - T[5] * T[7]) / det;
+      Ti[1] = (T[2] * T[7] - T[1] * T[8]) / det;
+      Ti[2] = (T[1] * T[5] - T[2] * T[4]) / det;
+      Ti[3] = (T[5] * T[6] - T[3] * T[8]) / det;
+      Ti[4] = (T[0] * T[8] - T[2] * T[6]) / det;
+      Ti[5] = (T[2] * T[3] - T[0] * T[5]) / det;
+      Ti[6] = (T[3] * T[7] - T[4] * T[6]) / det;
+      Ti[7] = (T[1] * T[6] - T[0] * T[7]) / det;
+      Ti[8] = (T[0] * T[4] - T[1] * T[3]) / det;
+    }
+    for (int side = 0; side < 2; side++) {
+      if (!has[side]) continue;
+      const double sg = side ? -1.0 : 1.0;
+      double X[9];  // sg T^-1 Rot, row-major
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+          double v = 0.0;
+          for (int k = 0; k < 3; k++) v += Ti[3 * i + k] * hc[18 * side + k + 3 * (3 + j)];
+          X[3 * i + j] = sg * v;
+        }
+      double* l = f + 9 + 3 * side;
+      l[0] = 0.5 * (X[7] - X[5]);
+      l[1] = 0.5 * (X[2] - X[6]);
+      l[2] = 0.5 * (X[3] - X[1]);
+    }
+    // rebuild as the kernels do and compare
+    double hr[36];
+    for (int side = 0; side < 2; side++) {
+      const double* l = f + 9 + 3 * side;
+      const double S[9] = {0.0, -l[2], l[1], l[2], 0.0, -l[0], -l[1], l[0], 0.0};
+      const double sg = side ? -1.0 : 1.0;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+          hr[18 * side + i + 3 * j] = sg * f[3 * i + j];
+          const double rot = f[3 * i] * S[j] + f[3 * i + 1] * S[3 + j] + f[3 * i + 2] * S[6 + j];
+          hr[18 * side + i + 3 * (3 + j)] = sg * rot;
+        }
+    }
+    for (int side = 0; side < 2; side++) {
+      if (!has[side]) continue;
+      for (int e = 0; e < 18; e++)
+        if (std::fabs(hr[18 * side + e] - hc[18 * side + e]) > 1e-13 * big) return false;
+    }
+    const int slot = orig2slot[(size_t)c], sl = slot / 64, lane = slot % 64;
+    for (int e = 0; e < 15; e++) (*Fs)[((size_t)sl * 15 + e) * 64 + (size_t)lane] = f[e];
+  }
+  return true;
+}
+
 static DevParams make_params(const fc3d_hip_params* p) {
   DevParams d;
   std::memset(&d, 0, sizeof d);
@@ -1395,7 +1483,8 @@ void fc3d_hip_problem_destroy(fc3d_hip_problem* h) {
   free_accel(h);
   void* dfp[] = {h->dep_ptr, h->dep_idx, h->qlist, h->done,    h->df,    h->d_block_dst,
                  h->d_o2s,    h->d_cps,    h->d_ba,   h->d_bb,   h->d_inc_ptr, h->d_inc, h->d_H,
-                 h->d_minv,   h->d_vfree,  h->d_V,    h->d_H32,     h->d_minv32, h->d_V32};
+                 h->d_minv,   h->d_vfree,  h->d_V,    h->d_H32,     h->d_minv32, h->d_V32,
+                 h->d_F,      h->d_F32};
   for (void* p : dfp)
     if (p) hipFree(p);
   if (h->stream) hipStreamDestroy(h->stream);
@@ -1670,33 +1759,51 @@ int fc3d_hip_problem_set_bodies(fc3d_hip_problem* h, int nbodies, const int* bod
     bb[(size_t)slot] = body_b[c];
     for (int e = 0; e < 36; e++) Hs[((size_t)sl * 36 + e) * 64 + (size_t)lane] = H[36 * (size_t)c + e];
   }
+  // frames layout (15 values a contact instead of 45, W_cc rebuilt in the
+  // sweep) only with FC3D_HIP_FRAMES=1 in the environment and rigid-body
+  // blocks on every contact: on MI355X the sweep is bound by its chain of
+  // dependent loads, not by bytes, and the rebuild's extra M^-1 round trip
+  // makes it 9-10% slower per sweep than the general layout
+  std::vector<double> Fs;
+  const char* fenv = std::getenv("FC3D_HIP_FRAMES");
+  const bool frames = fenv && fenv[0] == '1' &&
+                      rigid_frames(nc, body_a, body_b, H, h->orig2slot, h->nslices, &Fs);
   // drop earlier factors, upload the new ones
   drop_bodies(h);
   std::vector<double> mv(Minv, Minv + (size_t)nbodies * (minv_diagonal ? 6 : 36));
   std::vector<double> vf(vfree, vfree + 6 * (size_t)nbodies);
   if ((rc = dev_upload(&h->d_ba, ba)) || (rc = dev_upload(&h->d_bb, bb)) ||
       (rc = dev_upload(&h->d_inc_ptr, ptr)) || (rc = dev_upload(&h->d_inc, inc)) ||
-      (rc = dev_upload(&h->d_H, Hs)) || (rc = dev_upload(&h->d_minv, mv)) ||
-      (rc = dev_upload(&h->d_vfree, vf)) || (rc = dev_alloc(&h->d_V, 2 * (size_t)nbodies)))
+      (rc = frames ? dev_upload(&h->d_F, Fs) : dev_upload(&h->d_H, Hs)) ||
+      (rc = dev_upload(&h->d_minv, mv)) || (rc = dev_upload(&h->d_vfree, vf)) ||
+      (rc = dev_alloc(&h->d_V, 2 * (size_t)nbodies)))
     return rc;
+  h->frames = frames ? 1 : 0;
   h->nbodies = nbodies;
   h->minv_diag = minv_diagonal ? 1 : 0;
   // algorithmic bytes per contact of one velocity-form colour pass, in
-  // values of the sweep precision: H (36), W_cc (9), mu (1), r read and
-  // written (4 + 4, padded vec4), PLI step read and written (2); per body
-  // side present: its velocity read and written (8 + 8, two padded vec4) and
-  // M^-1 (6 or 36); plus len and the two body ids (12 bytes)
+  // values of the sweep precision: H (36) and W_cc (9), or in the frames
+  // layout T and the lever arms (15) with W_cc recomputed; mu (1), r read
+  // and written (4 + 4, padded vec4), PLI step read and written (2); per
+  // body side present: its velocity read and written (8 + 8, two padded
+  // vec4) and M^-1 (6 or 36); plus len and the two body ids (12 bytes)
   const double mb = minv_diagonal ? 6.0 : 36.0;
   h->vel_bytes64.assign((size_t)h->ncolours, 0.0);
   h->vel_bytes32.assign((size_t)h->ncolours, 0.0);
   for (int c = 0; c < nc; c++) {
     const int col = h->colour[(size_t)c];
     const double sides = (body_a[c] >= 0 ? 1.0 : 0.0) + (body_b[c] >= 0 ? 1.0 : 0.0);
-    const double vals = 36 + 9 + 1 + 8 + 2 + sides * (16 + mb);
+    const double vals = (frames ? 15 : 36 + 9) + 1 + 8 + 2 + sides * (16 + mb);
     h->vel_bytes64[(size_t)col] += 8.0 * vals + 12.0;
     h->vel_bytes32[(size_t)col] += 4.0 * vals + 12.0;
   }
   return 0;
+}
+
+int fc3d_hip_problem_body_layout(const fc3d_hip_problem* h) {
+  if (!h) return NUMERICS_ERR_NULL_POINTER;
+  if (!h->nbodies) return 0;
+  return h->frames ? 2 : 1;
 }
 
 int fc3d_hip_problem_features(fc3d_hip_problem* h, float* feat, float* target) {
@@ -1774,7 +1881,7 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
               p->accel_depth, AA_MAX, p->accel_every, p->accel_reg);
     return NUMERICS_ERR_INVALID_ARGUMENT;
   }
-  if (p->velocity_form && !h->d_H) {
+  if (p->velocity_form && !h->d_H && !h->d_F) {
     set_error("fc3d_hip_solve: velocity_form needs the factors of W "
               "(fc3d_hip_problem_set_bodies)");
     return NUMERICS_ERR_INVALID_OPTION;

@@ -171,6 +171,17 @@ __device__ __forceinline__ void load_slot(const SweepPtrs<T>& P, int s, int lane
   for (int e = 0; e < 3; e++) in->ql[e] = P.q[(size_t)(s * 3 + e) * 64 + lane];
 }
 
+// The slot's state without its diagonal block: mu, the old reaction and the
+// PLI step.
+template <typename T>
+__device__ __forceinline__ void load_slot_state(const SweepPtrs<T>& P, int s, int lane,
+                                                SlotInputs<T>* in) {
+  const int slot = s * 64 + lane;
+  in->lp.mu = P.mu[slot];
+  in->ro = P.r[slot];
+  in->pli = P.pli[slot];
+}
+
 // Local solve, relaxation, light-error term, optional filter and write-back
 // for a slot whose local q (q + sum_j W_ij r_j) is in in->ql.
 template <typename T>

@@ -10,6 +10,13 @@
 // Contacts of one colour share no body, so a colour's updates never collide.
 // Per contact this reads the 3x6 blocks of H and two body velocities instead
 // of the ~k neighbour blocks of W and k neighbour reactions.
+//
+// Frames layout: when every contact's blocks have the rigid-body form
+//     H_{c,a} = [ T | T S(l_a) ],   H_{c,b} = -[ T | T S(l_b) ]
+// (S(l) w = l x w; T the contact frame, l the lever arms, as the reference's
+// NewtonEuler3DR Jacobians are built), only T and the two lever arms are
+// stored (15 values instead of 36), the blocks are rebuilt in registers, and
+// W_cc = sum_k H_{c,k} M_k^-1 H_{c,k}^T is recomputed instead of read.
 #pragma once
 #include "fc3d_accel.hpp"
 
@@ -20,6 +27,7 @@ struct BodyPtrs {
   const int* ba;     // per slot: body of side a, -1 for none or padding
   const int* bb;     // per slot: body of side b
   const T* H;        // [slice][36][64]: side a entries 0-17, side b 18-35 (3x6 column-major)
+  const T* F;        // frames layout: [slice][15][64]: T row-major (0-8), l_a (9-11), l_b (12-14)
   const T* minv;     // per body: 6 diagonal entries, or 36 (6x6 column-major)
   int minv_diag;
   typename vec4<T>::type* V;  // per body: (v0, v1, v2), (v3, v4, v5)
@@ -46,9 +54,83 @@ __device__ __forceinline__ void minv_apply(const BodyPtrs<T>& B, int k, const T*
   }
 }
 
+// The two 3x6 blocks of a contact (column-major, side a then side b) from its
+// frame and lever arms: (T S(l))_{ij} = sum_k T_ik S(l)_kj.
+template <typename T>
+__device__ __forceinline__ void frames_to_h(const T* f, T* h) {
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    const T* l = f + 9 + 3 * side;
+    const T S[9] = {T(0), -l[2], l[1], l[2], T(0), -l[0], -l[1], l[0], T(0)};
+    const T sg = side ? T(-1) : T(1);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        h[18 * side + i + 3 * j] = sg * f[3 * i + j];
+        const T rot = f[3 * i] * S[j] + f[3 * i + 1] * S[3 + j] + f[3 * i + 2] * S[6 + j];
+        h[18 * side + i + 3 * (3 + j)] = sg * rot;
+      }
+    }
+  }
+}
+
+// Loads a slot's 3x6 blocks from either layout.
+template <typename T, bool FRAMES>
+__device__ __forceinline__ void load_h(const BodyPtrs<T>& B, int s, int lane, T* h) {
+  if (FRAMES) {
+    T f[15];
+#pragma unroll
+    for (int e = 0; e < 15; e++) f[e] = B.F[((size_t)s * 15 + e) * 64 + lane];
+    frames_to_h<T>(f, h);
+  } else {
+#pragma unroll
+    for (int e = 0; e < 36; e++) h[e] = B.H[((size_t)s * 36 + e) * 64 + lane];
+  }
+}
+
+// dv = M_k^-1 g with the body's M^-1 already in registers (diagonal: mk[6]),
+// or read from memory (6x6 blocks).
+template <typename T>
+__device__ __forceinline__ void minv_apply_r(const BodyPtrs<T>& B, int k, const T* mk,
+                                             const T* g, T* dv) {
+  if (B.minv_diag) {
+#pragma unroll
+    for (int l = 0; l < 6; l++) dv[l] = mk[l] * g[l];
+  } else {
+    minv_apply<T>(B, k, g, dv);
+  }
+}
+
+// W_cc = sum over the present sides of H M^-1 H^T (column-major 3x3).
+template <typename T>
+__device__ __forceinline__ void wcc_from_h(const BodyPtrs<T>& B, const int* kb, const T (*mk)[6],
+                                           const T* h, T* W) {
+#pragma unroll
+  for (int e = 0; e < 9; e++) W[e] = T(0);
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    if (kb[side] < 0) continue;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      T g[6], dv[6];
+#pragma unroll
+      for (int l = 0; l < 6; l++) g[l] = h[18 * side + j + 3 * l];
+      minv_apply_r<T>(B, kb[side], mk[side], g, dv);
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        T acc = T(0);
+#pragma unroll
+        for (int l = 0; l < 6; l++) acc += h[18 * side + i + 3 * l] * dv[l];
+        W[i + 3 * j] += acc;
+      }
+    }
+  }
+}
+
 // v_k = v_free,k + M_k^-1 sum_{c at k} H_{c,k}^T r_c for every body (in a fixed
 // order of its contacts).  With `as`, only after an Anderson step changed r.
-template <typename T>
+template <typename T, bool FRAMES>
 __global__ void k_vinit(BodyPtrs<T> B, const typename vec4<T>::type* __restrict__ r, int nbodies,
                         const SolveState* st, const AccelState* as, int want_phase) {
   if (st && (st->stop || st->phase != want_phase)) return;
@@ -60,11 +142,12 @@ __global__ void k_vinit(BodyPtrs<T> B, const typename vec4<T>::type* __restrict_
     const int e = B.inc[t], slot = e >> 1, side = e & 1;
     const int s = slot >> 6, lane = slot & 63;
     const typename vec4<T>::type rc = r[slot];
-    const T* h = B.H + (size_t)s * 36 * 64 + (size_t)(18 * side) * 64 + lane;
+    T hh[36];
+    load_h<T, FRAMES>(B, s, lane, hh);
+    const T* h = hh + 18 * side;
 #pragma unroll
     for (int l = 0; l < 6; l++)
-      g[l] += h[(size_t)(3 * l) * 64] * rc.x + h[(size_t)(3 * l + 1) * 64] * rc.y +
-              h[(size_t)(3 * l + 2) * 64] * rc.z;
+      g[l] += h[3 * l] * rc.x + h[3 * l + 1] * rc.y + h[3 * l + 2] * rc.z;
   }
   T dv[6];
   minv_apply<T>(B, k, g, dv);
@@ -85,7 +168,7 @@ __global__ void k_vinit(BodyPtrs<T> B, const typename vec4<T>::type* __restrict_
 // calling wave; writes the slice's partial sums.  With CHECK, the stop flag
 // is tested after the slot's loads are issued (the caller tests it
 // otherwise) and the function returns without writing if it is set.
-template <typename T, bool CHECK>
+template <typename T, bool CHECK, bool FRAMES>
 __device__ __forceinline__ void vel_slice(const SweepPtrs<T>& P, const BodyPtrs<T>& B,
                                           const DevParams* __restrict__ prm,
                                           const SolveState* __restrict__ st, int s, int lane,
@@ -99,25 +182,34 @@ __device__ __forceinline__ void vel_slice(const SweepPtrs<T>& P, const BodyPtrs<
   const int stop = CHECK ? st->stop : 0, phase = CHECK ? st->phase : want_phase;
   const int len = P.len[slot];
   SlotInputs<T> in;
-  load_slot<T>(P, s, lane, &in);
+  if (FRAMES)
+    load_slot_state<T>(P, s, lane, &in);  // W_cc is recomputed below
+  else
+    load_slot<T>(P, s, lane, &in);
   const int kb[2] = {B.ba[slot], B.bb[slot]};
   T h[36];
-#pragma unroll
-  for (int e = 0; e < 36; e++) h[e] = B.H[((size_t)s * 36 + e) * 64 + lane];
+  load_h<T, FRAMES>(B, s, lane, h);
   typename vec4<T>::type va[2][2];
+  // frames: W_cc is rebuilt from M^-1 before the local solve, so the
+  // diagonal M^-1 of both bodies is loaded here with V, not after the solve
+  T mk[2][6];
 #pragma unroll
   for (int side = 0; side < 2; side++) {
     const size_t k = kb[side] >= 0 ? (size_t)kb[side] : 0;
     va[side][0] = B.V[2 * k];
     va[side][1] = B.V[2 * k + 1];
+#pragma unroll
+    for (int l = 0; l < 6; l++) mk[side][l] = FRAMES && B.minv_diag ? B.minv[6 * k + l] : T(0);
   }
   if (CHECK) {
-    asm volatile("" ::"v"(len), "v"(in.lp.W[0]), "v"(in.lp.mu), "v"(in.ro.x), "v"(in.pli),
-                 "v"(h[0]), "v"(h[35]), "v"(va[0][0].x), "v"(va[1][1].x), "s"(stop), "s"(phase));
+    if (FRAMES) asm volatile("" ::"v"(mk[0][0]), "v"(mk[1][5]));
+    asm volatile("" ::"v"(len), "v"(in.lp.mu), "v"(in.ro.x), "v"(in.pli), "v"(h[0]), "v"(h[35]),
+                 "v"(va[0][0].x), "v"(va[1][1].x), "s"(stop), "s"(phase));
     if (stop || phase != want_phase) return;
   }
   if (len >= 0) {
     const T tol = T(want_phase ? prm->fp32_local_tol : st->local_tol);
+    if (FRAMES) wcc_from_h<T>(B, kb, mk, h, in.lp.W);
     // u = H_a v_a + H_b v_b, then q_loc = u - W_cc r_old
     T u[3] = {T(0), T(0), T(0)};
 #pragma unroll
@@ -146,7 +238,10 @@ __device__ __forceinline__ void vel_slice(const SweepPtrs<T>& P, const BodyPtrs<
       for (int l = 0; l < 6; l++)
         g[l] = h[18 * side + 3 * l] * dr[0] + h[18 * side + 3 * l + 1] * dr[1] +
                h[18 * side + 3 * l + 2] * dr[2];
-      minv_apply<T>(B, kb[side], g, dv);
+      if (FRAMES)
+        minv_apply_r<T>(B, kb[side], mk[side], g, dv);
+      else
+        minv_apply<T>(B, kb[side], g, dv);
       // the other side may be the same body: read back what this thread wrote
       typename vec4<T>::type v0 = B.V[2 * (size_t)kb[side]], v1 = B.V[2 * (size_t)kb[side] + 1];
       v0.x += dv[0]; v0.y += dv[1]; v0.z += dv[2];
@@ -163,9 +258,14 @@ __device__ __forceinline__ void vel_slice(const SweepPtrs<T>& P, const BodyPtrs<
   }
 }
 
-// One colour of one sweep in the velocity form: one wave per slice.
-template <typename T>
-__global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void k_sweep_vel(SweepPtrs<T> P,
+// One colour of one sweep in the velocity form: one wave per slice.  A wave
+// waits on a chain of dependent loads (slot state, body ids, V), so waves in
+// flight matter more than registers: the FP32 kernel is held to 168 VGPRs
+// (3 waves/SIMD, no spills), 7% faster a sweep than at 2 waves/SIMD; the
+// FP64 one to 256 (2 waves/SIMD, ~200 B a lane spilled), 9% faster than at
+// 1 wave/SIMD with no spills.
+template <typename T, bool FRAMES>
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 2 : 3) void k_sweep_vel(SweepPtrs<T> P,
                                                                           BodyPtrs<T> B,
                                                       const DevParams* __restrict__ prm,
                                                       const SolveState* __restrict__ st,
@@ -173,7 +273,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void k_sweep_vel(Sweep
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int s = __builtin_amdgcn_readfirstlane(s_begin + xcd_block() * 4 + wave);
   if (s >= s_end) return;
-  vel_slice<T, true>(P, B, prm, st, s, lane, want_phase);
+  vel_slice<T, true, FRAMES>(P, B, prm, st, s, lane, want_phase);
 }
 
 // The last colours of a sweep, each of at most two rounds of the block's
@@ -185,7 +285,7 @@ constexpr int vel_tail_threads() {
   return sizeof(T) == 8 ? 256 : 512;  // the waves k_sweep_vel's registers allow per CU
 }
 
-template <typename T>
+template <typename T, bool FRAMES>
 __global__ __launch_bounds__(vel_tail_threads<T>()) void k_sweep_vel_tail(
     SweepPtrs<T> P, BodyPtrs<T> B, const DevParams* __restrict__ prm,
     const SolveState* __restrict__ st, const int* __restrict__ colour_slices, int c_begin,
@@ -195,7 +295,8 @@ __global__ __launch_bounds__(vel_tail_threads<T>()) void k_sweep_vel_tail(
   constexpr int waves = vel_tail_threads<T>() / 64;
   for (int c = c_begin; c < c_end; c++) {
     const int sb = colour_slices[c], se = colour_slices[c + 1];
-    for (int s = sb + wave; s < se; s += waves) vel_slice<T, false>(P, B, prm, st, s, lane, want_phase);
+    for (int s = sb + wave; s < se; s += waves)
+      vel_slice<T, false, FRAMES>(P, B, prm, st, s, lane, want_phase);
     __syncthreads();  // the next colour reads this colour's body velocities
   }
 }

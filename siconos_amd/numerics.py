@@ -480,6 +480,10 @@ class HipProblem:
         self._check(self._lib.fc3d_hip_problem_update(
             self._h, *[None if v is None else dptr(v) for v in a]), "update")
 
+    def body_layout(self):
+        """0 no factors, 1 general 3x6 blocks, 2 frames layout."""
+        return self._lib.fc3d_hip_problem_body_layout(self._h)
+
     def set_bodies(self, nbodies, body_a, body_b, H, minv, vfree, minv_diagonal=True):
         """Factors of W for velocity-form sweeps (fc3d_hip_problem_set_bodies)."""
         a = np.ascontiguousarray(body_a, dtype=np.int32)

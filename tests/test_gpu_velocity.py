@@ -169,3 +169,52 @@ def test_tail_launch_gives_the_same_iterates(lib, precision):
     i = H.solve(N.hip_params(tol=1e-300, max_iter=5, warm_start=1, velocity_form=1,
                              precision=precision, fp32_tol=1e-300, instrument=1))
     assert i.sweep_kernel_launches < 5 * H.num_colours
+
+
+def test_rigid_body_blocks_use_the_frames_layout_and_others_do_not(oracle):
+    """With FC3D_HIP_FRAMES=1, generator piles (rigid-body blocks) take the
+    frames layout (W_cc recomputed); random 3x6 blocks keep the general
+    layout, and so does every problem by default.  Both track the rows-of-W
+    iterates (the oracle's) to rounding."""
+    import os
+    g = N.generate_granular_pile(3000, 8000, 0.1, 0.9, 4)
+    assert _with_bodies(g).body_layout() == 1
+    os.environ["FC3D_HIP_FRAMES"] = "1"
+    try:
+        H = _with_bodies(g)
+    finally:
+        del os.environ["FC3D_HIP_FRAMES"]
+    assert H.body_layout() == 2
+    rng = np.random.default_rng(11)
+    nb, nc = 300, 1000
+    pairs = set()
+    while len(pairs) < nc:
+        a, b = sorted(rng.choice(nb, 2, replace=False).tolist())
+        pairs.add((a, b))
+    pairs = sorted(pairs)
+    ba = np.array([p[0] for p in pairs], dtype=np.int32)
+    bb = np.array([p[1] for p in pairs], dtype=np.int32)
+    Hr = rng.normal(size=36 * nc)
+    minv = rng.uniform(0.5, 2.0, size=6 * nb)
+    vfree = rng.normal(size=6 * nb)
+    mu = rng.uniform(0.1, 0.9, size=nc)
+    W, q, _ = N.assemble(nb, ba, bb, Hr, minv, vfree)
+    os.environ["FC3D_HIP_FRAMES"] = "1"
+    try:
+        Pr = N.HipProblem(W, q, mu)
+        Pr.set_bodies(nb, ba, bb, Hr, minv, vfree)
+        assert Pr.body_layout() == 1
+    finally:
+        del os.environ["FC3D_HIP_FRAMES"]
+    for prob, kind in ((g, "pile"), (None, "random")):
+        if prob is None:
+            P = Pr
+            WW, qq, mm = W, q, mu
+        else:
+            P, WW, qq, mm = H, g.W, g.q, g.mu
+        for K in (3, 20):
+            P.solve(N.hip_params(tol=1e-300, max_iter=K, warm_start=1, velocity_form=1))
+            rv = P.reaction()
+            info, ro, uo, it, res = oracle.nsgs(WW, qq, mm, tol=1e-300, max_iter=K,
+                                                order=P.order())
+            assert np.linalg.norm(rv - ro) <= 1e-10 * np.linalg.norm(ro), (kind, K)
