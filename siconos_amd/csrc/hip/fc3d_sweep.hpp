@@ -65,9 +65,46 @@ struct SweepPtrs {
   typename vec4<T>::type* f = nullptr;  // if set: r_new - r_old of the sweep
 };
 
+// Adds the value of the lane that DPP control CTRL selects (all lanes active).
+template <int CTRL>
+__device__ __forceinline__ double dpp_add(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_update_dpp(0u, (unsigned)u, CTRL, 0xF, 0xF, false);
+  const unsigned hi = __builtin_amdgcn_update_dpp(0u, (unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+  return v + __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Adds the value of the lane ROWS lanes away (lane ^ ROWS, ROWS = 16 or 32)
+// with gfx950's v_permlane{16,32}_swap.  Swapping v with itself leaves each
+// lane holding its own value in one result and its partner's in the other
+// (which one depends on the lane), and IEEE addition is commutative, so the
+// sum of the two is v + v[lane ^ ROWS] bit for bit.
+template <int ROWS>
+__device__ __forceinline__ double swap_add(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned ul = (unsigned)u, uh = (unsigned)(u >> 32);
+  const auto lo = ROWS == 16 ? __builtin_amdgcn_permlane16_swap(ul, ul, false, false)
+                             : __builtin_amdgcn_permlane32_swap(ul, ul, false, false);
+  const auto hi = ROWS == 16 ? __builtin_amdgcn_permlane16_swap(uh, uh, false, false)
+                             : __builtin_amdgcn_permlane32_swap(uh, uh, false, false);
+  const double a = __builtin_bit_cast(double, ((unsigned long long)hi[0] << 32) | lo[0]);
+  const double b = __builtin_bit_cast(double, ((unsigned long long)hi[1] << 32) | lo[1]);
+  return a + b;
+}
+
+// Sum over the wave's 64 lanes as the xor butterfly (partners lane ^ 1, ^ 2,
+// ^ 4, ^ 8, ^ 16, ^ 32) computes it, every lane ending with the same bits,
+// with no LDS round trip: quad permutations for ^ 1 and ^ 2; after those
+// steps each quad holds one value, so the half-row mirror (lane 7 - i of each
+// 8) reads what lane ^ 4 holds, and likewise the row mirror for ^ 8; the
+// permlane swaps for ^ 16 and ^ 32.  Call with every lane active.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  v = dpp_add<0xB1>(v);   // quad_perm [1,0,3,2]
+  v = dpp_add<0x4E>(v);   // quad_perm [2,3,0,1]
+  v = dpp_add<0x141>(v);  // row_half_mirror
+  v = dpp_add<0x140>(v);  // row_mirror
+  v = swap_add<16>(v);
+  v = swap_add<32>(v);
   return v;
 }
 
@@ -159,11 +196,21 @@ struct SlotInputs {
   T ql[3];
 };
 
-template <typename T>
+// A load that marks its line non-temporal when NT: streamed planes then leave
+// the body data that later colours reread in L2.
+template <bool NT, typename T>
+__device__ __forceinline__ T ld(const T* p) {
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+
+template <typename T, bool NT = false>
 __device__ __forceinline__ void load_slot(const SweepPtrs<T>& P, int s, int lane, SlotInputs<T>* in) {
   const int slot = s * 64 + lane;
 #pragma unroll
-  for (int e = 0; e < 9; e++) in->lp.W[e] = P.diag[(size_t)(s * 9 + e) * 64 + lane];
+  for (int e = 0; e < 9; e++) in->lp.W[e] = ld<NT>(&P.diag[(size_t)(s * 9 + e) * 64 + lane]);
   in->lp.mu = P.mu[slot];
   in->ro = P.r[slot];
   in->pli = P.pli[slot];
@@ -182,13 +229,44 @@ __device__ __forceinline__ void load_slot_state(const SweepPtrs<T>& P, int s, in
   in->pli = P.pli[slot];
 }
 
+// The constants of the per-contact step, read from DevParams once per wave
+// (all wave-uniform, so they sit in scalar registers).
+struct StepConsts {
+  LocalParams lp;  // the local solver of this precision's sweeps
+  int rho_strategy;
+  int relaxation;
+  int filter;
+  double rho_constant;
+  double omega;
+};
+
+template <typename T>
+__device__ __forceinline__ StepConsts step_consts(const DevParams* __restrict__ prm) {
+  StepConsts c;
+  c.lp = sizeof(T) == 8 ? prm->lp : prm->lp32;
+  c.rho_strategy = prm->rho_strategy;
+  c.relaxation = prm->relaxation;
+  c.filter = prm->filter;
+  c.rho_constant = prm->rho_constant;
+  c.omega = prm->omega;
+  return c;
+}
+
+// Makes every field of c live here, so that its scalar loads are issued
+// together before this point instead of one by one where each is first used.
+__device__ __forceinline__ void step_consts_ready(const StepConsts& c) {
+  asm volatile("" ::"s"(c.lp.solver), "s"(c.lp.newton_max_iter), "s"(c.lp.formulation),
+               "s"(c.lp.ls_max_iter), "s"(c.lp.hybrid_strategy), "s"(c.lp.hybrid_max_loop),
+               "s"(c.lp.hybrid_max_iter), "s"(c.rho_strategy), "s"(c.relaxation), "s"(c.filter),
+               "s"(c.rho_constant), "s"(c.omega));
+}
+
 // Local solve, relaxation, light-error term, optional filter and write-back
 // for a slot whose local q (q + sum_j W_ij r_j) is in in->ql.
 template <typename T>
-__device__ __forceinline__ void finish_slot(const SweepPtrs<T>& P, const DevParams* __restrict__ prm,
-                                            int s, int lane, T tol, SlotInputs<T>& in,
-                                            double* light, double* nr2,
-                                            typename vec4<T>::type* out_r = nullptr) {
+__device__ __forceinline__ void finish_slot(const SweepPtrs<T>& P, const StepConsts& c, int s,
+                                            int lane, T tol, SlotInputs<T>& in, double* light,
+                                            double* nr2, typename vec4<T>::type* out_r = nullptr) {
   const int slot = s * 64 + lane;
   LocalProblem<T>& lp = in.lp;
   const typename vec4<T>::type ro = in.ro;
@@ -197,17 +275,17 @@ __device__ __forceinline__ void finish_slot(const SweepPtrs<T>& P, const DevPara
   const T rold[3] = {ro.x, ro.y, ro.z};
   T lr[3] = {ro.x, ro.y, ro.z};
   T rho[3];
-  compute_rho<T>(prm->rho_strategy, T(prm->rho_constant), lp.W, rho);
+  compute_rho<T>(c.rho_strategy, T(c.rho_constant), lp.W, rho);
   T res = T(0);
-  local_solve<T>(lp, lr, sizeof(T) == 8 ? prm->lp : prm->lp32, tol, &pli, rho, &res);
-  if (prm->relaxation) {
-    const T om = T(prm->omega);
+  local_solve<T>(lp, lr, c.lp, tol, &pli, rho, &res);
+  if (c.relaxation) {
+    const T om = T(c.omega);
 #pragma unroll
     for (int e = 0; e < 3; e++) lr[e] = om * lr[e] + (T(1) - om) * rold[e];
   }
   const T d0 = rold[0] - lr[0], d1 = rold[1] - lr[1], d2 = rold[2] - lr[2];
   *light = double(d0 * d0 + d1 * d1 + d2 * d2);
-  const bool accept = !(prm->filter && (isnan(res) || isinf(res) || res > T(1)));
+  const bool accept = !(c.filter && (isnan(res) || isinf(res) || res > T(1)));
   typename vec4<T>::type out = ro;
   if (accept) { out.x = lr[0]; out.y = lr[1]; out.z = lr[2]; }
   P.r[slot] = out;
@@ -221,6 +299,14 @@ __device__ __forceinline__ void finish_slot(const SweepPtrs<T>& P, const DevPara
   *nr2 = sizeof(T) == 8 ? double(out.x * out.x + out.y * out.y + out.z * out.z)
                         : double(out.x) * double(out.x) + double(out.y) * double(out.y) +
                               double(out.z) * double(out.z);
+}
+
+template <typename T>
+__device__ __forceinline__ void finish_slot(const SweepPtrs<T>& P, const DevParams* __restrict__ prm,
+                                            int s, int lane, T tol, SlotInputs<T>& in,
+                                            double* light, double* nr2,
+                                            typename vec4<T>::type* out_r = nullptr) {
+  finish_slot<T>(P, step_consts<T>(prm), s, lane, tol, in, light, nr2, out_r);
 }
 
 // The per-contact step of a sweep (fc3d_nsgs.c:737-822) for the lane's slot of

@@ -75,17 +75,17 @@ __device__ __forceinline__ void frames_to_h(const T* f, T* h) {
   }
 }
 
-// Loads a slot's 3x6 blocks from either layout.
-template <typename T, bool FRAMES>
+// Loads a slot's 3x6 blocks from either layout (non-temporal with NT).
+template <typename T, bool FRAMES, bool NT = false>
 __device__ __forceinline__ void load_h(const BodyPtrs<T>& B, int s, int lane, T* h) {
   if (FRAMES) {
     T f[15];
 #pragma unroll
-    for (int e = 0; e < 15; e++) f[e] = B.F[((size_t)s * 15 + e) * 64 + lane];
+    for (int e = 0; e < 15; e++) f[e] = ld<NT>(&B.F[((size_t)s * 15 + e) * 64 + lane]);
     frames_to_h<T>(f, h);
   } else {
 #pragma unroll
-    for (int e = 0; e < 36; e++) h[e] = B.H[((size_t)s * 36 + e) * 64 + lane];
+    for (int e = 0; e < 36; e++) h[e] = ld<NT>(&B.H[((size_t)s * 36 + e) * 64 + lane]);
   }
 }
 
@@ -180,15 +180,20 @@ __device__ __forceinline__ void vel_slice(const SweepPtrs<T>& P, const BodyPtrs<
   // slots read body 0).  The empty asm makes the loaded values live before
   // the test, which keeps the compiler from sinking the loads below it.
   const int stop = CHECK ? st->stop : 0, phase = CHECK ? st->phase : want_phase;
+  const StepConsts sc = step_consts<T>(prm);
+  const T tol = T(want_phase ? prm->fp32_local_tol : st->local_tol);
   const int len = P.len[slot];
   SlotInputs<T> in;
+  // FP64: the streamed planes are loaded non-temporal, which keeps V and M^-1
+  // in L2 (5% a sweep); in FP32 it costs 2%
+  constexpr bool NT = sizeof(T) == 8;
   if (FRAMES)
     load_slot_state<T>(P, s, lane, &in);  // W_cc is recomputed below
   else
-    load_slot<T>(P, s, lane, &in);
+    load_slot<T, NT>(P, s, lane, &in);
   const int kb[2] = {B.ba[slot], B.bb[slot]};
   T h[36];
-  load_h<T, FRAMES>(B, s, lane, h);
+  load_h<T, FRAMES, NT>(B, s, lane, h);
   typename vec4<T>::type va[2][2];
   // frames: W_cc is rebuilt from M^-1 before the local solve, so the
   // diagonal M^-1 of both bodies is loaded here with V, not after the solve
@@ -201,6 +206,8 @@ __device__ __forceinline__ void vel_slice(const SweepPtrs<T>& P, const BodyPtrs<
 #pragma unroll
     for (int l = 0; l < 6; l++) mk[side][l] = FRAMES && B.minv_diag ? B.minv[6 * k + l] : T(0);
   }
+  step_consts_ready(sc);
+  asm volatile("" ::"s"(tol));
   if (CHECK) {
     if (FRAMES) asm volatile("" ::"v"(mk[0][0]), "v"(mk[1][5]));
     asm volatile("" ::"v"(len), "v"(in.lp.mu), "v"(in.ro.x), "v"(in.pli), "v"(h[0]), "v"(h[35]),
@@ -208,7 +215,6 @@ __device__ __forceinline__ void vel_slice(const SweepPtrs<T>& P, const BodyPtrs<
     if (stop || phase != want_phase) return;
   }
   if (len >= 0) {
-    const T tol = T(want_phase ? prm->fp32_local_tol : st->local_tol);
     if (FRAMES) wcc_from_h<T>(B, kb, mk, h, in.lp.W);
     // u = H_a v_a + H_b v_b, then q_loc = u - W_cc r_old
     T u[3] = {T(0), T(0), T(0)};
@@ -228,26 +234,62 @@ __device__ __forceinline__ void vel_slice(const SweepPtrs<T>& P, const BodyPtrs<
 #pragma unroll
     for (int i = 0; i < 3; i++) in.ql[i] = u[i] - wr[i];
     typename vec4<T>::type out;
-    finish_slot<T>(P, prm, s, lane, tol, in, &light, &nr2, &out);
+    finish_slot<T>(P, sc, s, lane, tol, in, &light, &nr2, &out);
     const T dr[3] = {out.x - rold[0], out.y - rold[1], out.z - rold[2]};
+    if constexpr (sizeof(T) == 8) {
+      // FP64: keeping the loaded V live through the local solve costs more
+      // spills than reading it back (the other side may be the same body)
 #pragma unroll
-    for (int side = 0; side < 2; side++) {
-      if (kb[side] < 0) continue;
-      T g[6], dv[6];
+      for (int side = 0; side < 2; side++) {
+        if (kb[side] < 0) continue;
+        T g[6], dv[6];
 #pragma unroll
-      for (int l = 0; l < 6; l++)
-        g[l] = h[18 * side + 3 * l] * dr[0] + h[18 * side + 3 * l + 1] * dr[1] +
-               h[18 * side + 3 * l + 2] * dr[2];
-      if (FRAMES)
-        minv_apply_r<T>(B, kb[side], mk[side], g, dv);
-      else
-        minv_apply<T>(B, kb[side], g, dv);
-      // the other side may be the same body: read back what this thread wrote
-      typename vec4<T>::type v0 = B.V[2 * (size_t)kb[side]], v1 = B.V[2 * (size_t)kb[side] + 1];
-      v0.x += dv[0]; v0.y += dv[1]; v0.z += dv[2];
-      v1.x += dv[3]; v1.y += dv[4]; v1.z += dv[5];
-      B.V[2 * (size_t)kb[side]] = v0;
-      B.V[2 * (size_t)kb[side] + 1] = v1;
+        for (int l = 0; l < 6; l++)
+          g[l] = h[18 * side + 3 * l] * dr[0] + h[18 * side + 3 * l + 1] * dr[1] +
+                 h[18 * side + 3 * l + 2] * dr[2];
+        if (FRAMES)
+          minv_apply_r<T>(B, kb[side], mk[side], g, dv);
+        else
+          minv_apply<T>(B, kb[side], g, dv);
+        typename vec4<T>::type v0 = B.V[2 * (size_t)kb[side]], v1 = B.V[2 * (size_t)kb[side] + 1];
+        v0.x += dv[0]; v0.y += dv[1]; v0.z += dv[2];
+        v1.x += dv[3]; v1.y += dv[4]; v1.z += dv[5];
+        B.V[2 * (size_t)kb[side]] = v0;
+        B.V[2 * (size_t)kb[side] + 1] = v1;
+      }
+    } else {
+      // FP32: both sides' M^-1 H^T dr first, so their M^-1 loads are in flight
+      // together, and no V read-back
+      T dv[2][6];
+#pragma unroll
+      for (int side = 0; side < 2; side++) {
+        if (kb[side] < 0) continue;
+        T g[6];
+#pragma unroll
+        for (int l = 0; l < 6; l++)
+          g[l] = h[18 * side + 3 * l] * dr[0] + h[18 * side + 3 * l + 1] * dr[1] +
+                 h[18 * side + 3 * l + 2] * dr[2];
+        if (FRAMES)
+          minv_apply_r<T>(B, kb[side], mk[side], g, dv[side]);
+        else
+          minv_apply<T>(B, kb[side], g, dv[side]);
+      }
+      // a colour's contacts share no body, so V still holds what was loaded
+      // above; when both sides name the same body, side b adds to side a's sum
+      typename vec4<T>::type vn[2][2];
+#pragma unroll
+      for (int side = 0; side < 2; side++) {
+        if (kb[side] < 0) continue;
+        const bool again = side == 1 && kb[1] == kb[0];
+        typename vec4<T>::type v0 = again ? vn[0][0] : va[side][0];
+        typename vec4<T>::type v1 = again ? vn[0][1] : va[side][1];
+        v0.x += dv[side][0]; v0.y += dv[side][1]; v0.z += dv[side][2];
+        v1.x += dv[side][3]; v1.y += dv[side][4]; v1.z += dv[side][5];
+        vn[side][0] = v0;
+        vn[side][1] = v1;
+        B.V[2 * (size_t)kb[side]] = v0;
+        B.V[2 * (size_t)kb[side] + 1] = v1;
+      }
     }
   }
   light = wave_sum(light);
