@@ -106,6 +106,12 @@ typedef struct fc3d_hip_stats {
   double sweep_kernel_ms_fp32;
   int64_t sweep_kernel_launches_fp32;
   double sweep_kernel_bytes_fp32;
+  /* instrumented, velocity form: the part of the three *_fp32 figures spent
+   * in tail launches (the last small colours of a sweep in one workgroup);
+   * the rest is one launch per colour */
+  double sweep_tail_ms_fp32;
+  int64_t sweep_tail_launches_fp32;
+  double sweep_tail_bytes_fp32;
 } fc3d_hip_stats;
 
 FC3D_API int fc3d_hip_device_count(void);

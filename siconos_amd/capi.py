@@ -102,7 +102,9 @@ class HipStats(C.Structure):
                 ("accel_steps", C.c_int), ("accel_restarts", C.c_int),
                 ("accel_rejects", C.c_int), ("sweep_kernel_ms_fp32", C.c_double),
                 ("sweep_kernel_launches_fp32", C.c_int64),
-                ("sweep_kernel_bytes_fp32", C.c_double)]
+                ("sweep_kernel_bytes_fp32", C.c_double),
+                ("sweep_tail_ms_fp32", C.c_double), ("sweep_tail_launches_fp32", C.c_int64),
+                ("sweep_tail_bytes_fp32", C.c_double)]
 
 
 class Fc2dParams(C.Structure):

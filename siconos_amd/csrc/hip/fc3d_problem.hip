@@ -2027,6 +2027,11 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
             s->sweep_kernel_ms_fp32 += ms;
             s->sweep_kernel_launches_fp32 += 1;
             s->sweep_kernel_bytes_fp32 += by;
+            if (h->vel_on && c >= vel_tail_begin(h, phase)) {
+              s->sweep_tail_ms_fp32 += ms;
+              s->sweep_tail_launches_fp32 += 1;
+              s->sweep_tail_bytes_fp32 += by;
+            }
           }
         }
       } else if (no_graph) {

@@ -169,6 +169,11 @@ def test_tail_launch_gives_the_same_iterates(lib, precision):
     i = H.solve(N.hip_params(tol=1e-300, max_iter=5, warm_start=1, velocity_form=1,
                              precision=precision, fp32_tol=1e-300, instrument=1))
     assert i.sweep_kernel_launches < 5 * H.num_colours
+    if precision:
+        # one tail launch a sweep, reported apart from the per-colour launches
+        assert i.sweep_tail_launches_fp32 == i.fp32_sweeps == 5
+        assert 0 < i.sweep_tail_ms_fp32 < i.sweep_kernel_ms_fp32
+        assert 0 < i.sweep_tail_bytes_fp32 < i.sweep_kernel_bytes_fp32
 
 
 def test_rigid_body_blocks_use_the_frames_layout_and_others_do_not(oracle):
