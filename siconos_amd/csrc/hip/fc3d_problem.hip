@@ -238,38 +238,42 @@ __global__ __launch_bounds__(256) void k_residual(SweepPtrs<double> P, const Sol
 
 // Fixed-order sums of NV interleaved partials (part[i * NV + j], j < NV) with
 // one block of 1024 threads: thread t adds entries t, t + 1024, ... in that
-// order (eight loads in flight at a time), then waves and the 16 wave sums are
-// added in a fixed order.  Results are valid on thread 0.
+// order (sixteen loads in flight at a time, so a sweep's ~16k slice partials
+// cost one memory round trip), then waves and the 16 wave sums are added in a
+// fixed order.  Results are valid on thread 0.
 template <int NV>
 __device__ void block_sums(const double* __restrict__ part, int n, double* sh, double* out) {
   double v[NV];
 #pragma unroll
   for (int j = 0; j < NV; j++) v[j] = 0.0;
-  int i = threadIdx.x;
-  for (; i + 7 * 1024 < n; i += 8 * 1024) {
-    double t[8][NV];
+  for (int i = threadIdx.x; i < n; i += 16 * 1024) {
+    double t[16][NV];
 #pragma unroll
-    for (int u = 0; u < 8; u++)
+    for (int u = 0; u < 16; u++) {
+      const int k = i + u * 1024;
 #pragma unroll
-      for (int j = 0; j < NV; j++) t[u][j] = part[(size_t)(i + u * 1024) * NV + j];
+      for (int j = 0; j < NV; j++) t[u][j] = k < n ? part[(size_t)k * NV + j] : 0.0;
+    }
 #pragma unroll
-    for (int u = 0; u < 8; u++)
+    for (int u = 0; u < 16; u++)
 #pragma unroll
-      for (int j = 0; j < NV; j++) v[j] += t[u][j];
+      for (int j = 0; j < NV; j++)
+        if (i + u * 1024 < n) v[j] += t[u][j];
   }
-  for (; i < n; i += 1024)
-#pragma unroll
-    for (int j = 0; j < NV; j++) v[j] += part[(size_t)i * NV + j];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double w[NV];
 #pragma unroll
-  for (int j = 0; j < NV; j++) {
-    const double w = wave_sum(v[j]);
-    __syncthreads();
-    if (lane == 0) sh[wave] = w;
-    __syncthreads();
-    if (threadIdx.x == 0) {
+  for (int j = 0; j < NV; j++) w[j] = wave_sum(v[j]);
+  __syncthreads();  // sh may hold an earlier call's sums
+  if (lane == 0)
+#pragma unroll
+    for (int j = 0; j < NV; j++) sh[j * 16 + wave] = w[j];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
       double t = 0.0;
-      for (int k = 0; k < 16; k++) t += sh[k];
+      for (int k = 0; k < 16; k++) t += sh[j * 16 + k];
       out[j] = t;
     }
   }
@@ -280,7 +284,7 @@ __device__ void block_sums(const double* __restrict__ part, int n, double* sh, d
 __global__ __launch_bounds__(1024) void k_finalize_sweep(const DevParams* prm, SolveState* st,
                                                          const double* part, int nslices,
                                                          int want_phase) {
-  __shared__ double sh[16];
+  __shared__ double sh[2 * 16];
   if (st->stop || st->phase != want_phase) return;
   double sums[2];
   block_sums<2>(part, nslices, sh, sums);
@@ -327,7 +331,7 @@ __global__ __launch_bounds__(1024) void k_finalize_sweep(const DevParams* prm, S
 __global__ __launch_bounds__(1024) void k_finalize_residual(const DevParams* prm, SolveState* st,
                                                             const double* part, int nslices,
                                                             int plain) {
-  __shared__ double sh[16];
+  __shared__ double sh[3 * 16];
   if (!plain && !st->need_full) return;
   double sums[3];
   block_sums<3>(part, nslices, sh, sums);

@@ -236,31 +236,38 @@ __device__ __forceinline__ void vel_slice(const SweepPtrs<T>& P, const BodyPtrs<
     typename vec4<T>::type out;
     finish_slot<T>(P, sc, s, lane, tol, in, &light, &nr2, &out);
     const T dr[3] = {out.x - rold[0], out.y - rold[1], out.z - rold[2]};
-    if constexpr (sizeof(T) == 8) {
-      // FP64: keeping the loaded V live through the local solve costs more
-      // spills than reading it back (the other side may be the same body)
+    // dv = M^-1 H^T dr for both sides.  With diagonal M^-1 both sides'
+    // entries are loaded at once, so the update waits on one round trip.
+    // FP32 keeps the body velocities loaded before the solve; FP64 reads
+    // them back with M^-1 (keeping them live through its solve spills more).
+    T dv[2][6];
+    typename vec4<T>::type vb[2][2];
+    if (B.minv_diag) {
+      T m[2][6];
+#pragma unroll
+      for (int side = 0; side < 2; side++) {
+        const size_t k = kb[side] >= 0 ? (size_t)kb[side] : 0;
+#pragma unroll
+        for (int l = 0; l < 6; l++) m[side][l] = FRAMES ? mk[side][l] : B.minv[6 * k + l];
+        if constexpr (sizeof(T) == 8) {
+          vb[side][0] = B.V[2 * k];
+          vb[side][1] = B.V[2 * k + 1];
+        } else {
+          vb[side][0] = va[side][0];
+          vb[side][1] = va[side][1];
+        }
+      }
 #pragma unroll
       for (int side = 0; side < 2; side++) {
         if (kb[side] < 0) continue;
-        T g[6], dv[6];
 #pragma unroll
-        for (int l = 0; l < 6; l++)
-          g[l] = h[18 * side + 3 * l] * dr[0] + h[18 * side + 3 * l + 1] * dr[1] +
-                 h[18 * side + 3 * l + 2] * dr[2];
-        if (FRAMES)
-          minv_apply_r<T>(B, kb[side], mk[side], g, dv);
-        else
-          minv_apply<T>(B, kb[side], g, dv);
-        typename vec4<T>::type v0 = B.V[2 * (size_t)kb[side]], v1 = B.V[2 * (size_t)kb[side] + 1];
-        v0.x += dv[0]; v0.y += dv[1]; v0.z += dv[2];
-        v1.x += dv[3]; v1.y += dv[4]; v1.z += dv[5];
-        B.V[2 * (size_t)kb[side]] = v0;
-        B.V[2 * (size_t)kb[side] + 1] = v1;
+        for (int l = 0; l < 6; l++) {
+          const T g = h[18 * side + 3 * l] * dr[0] + h[18 * side + 3 * l + 1] * dr[1] +
+                      h[18 * side + 3 * l + 2] * dr[2];
+          dv[side][l] = m[side][l] * g;
+        }
       }
     } else {
-      // FP32: both sides' M^-1 H^T dr first, so their M^-1 loads are in flight
-      // together, and no V read-back
-      T dv[2][6];
 #pragma unroll
       for (int side = 0; side < 2; side++) {
         if (kb[side] < 0) continue;
@@ -269,27 +276,32 @@ __device__ __forceinline__ void vel_slice(const SweepPtrs<T>& P, const BodyPtrs<
         for (int l = 0; l < 6; l++)
           g[l] = h[18 * side + 3 * l] * dr[0] + h[18 * side + 3 * l + 1] * dr[1] +
                  h[18 * side + 3 * l + 2] * dr[2];
-        if (FRAMES)
-          minv_apply_r<T>(B, kb[side], mk[side], g, dv[side]);
-        else
-          minv_apply<T>(B, kb[side], g, dv[side]);
+        minv_apply<T>(B, kb[side], g, dv[side]);
+        if constexpr (sizeof(T) == 8) {
+          vb[side][0] = B.V[2 * (size_t)kb[side]];
+          vb[side][1] = B.V[2 * (size_t)kb[side] + 1];
+        } else {
+          vb[side][0] = va[side][0];
+          vb[side][1] = va[side][1];
+        }
       }
-      // a colour's contacts share no body, so V still holds what was loaded
-      // above; when both sides name the same body, side b adds to side a's sum
-      typename vec4<T>::type vn[2][2];
+    }
+    // a colour's contacts share no body, so vb holds the bodies' current
+    // velocities; when both sides name the same body, side b adds to side
+    // a's sum
+    typename vec4<T>::type vn[2][2];
 #pragma unroll
-      for (int side = 0; side < 2; side++) {
-        if (kb[side] < 0) continue;
-        const bool again = side == 1 && kb[1] == kb[0];
-        typename vec4<T>::type v0 = again ? vn[0][0] : va[side][0];
-        typename vec4<T>::type v1 = again ? vn[0][1] : va[side][1];
-        v0.x += dv[side][0]; v0.y += dv[side][1]; v0.z += dv[side][2];
-        v1.x += dv[side][3]; v1.y += dv[side][4]; v1.z += dv[side][5];
-        vn[side][0] = v0;
-        vn[side][1] = v1;
-        B.V[2 * (size_t)kb[side]] = v0;
-        B.V[2 * (size_t)kb[side] + 1] = v1;
-      }
+    for (int side = 0; side < 2; side++) {
+      if (kb[side] < 0) continue;
+      const bool again = side == 1 && kb[1] == kb[0];
+      typename vec4<T>::type v0 = again ? vn[0][0] : vb[side][0];
+      typename vec4<T>::type v1 = again ? vn[0][1] : vb[side][1];
+      v0.x += dv[side][0]; v0.y += dv[side][1]; v0.z += dv[side][2];
+      v1.x += dv[side][3]; v1.y += dv[side][4]; v1.z += dv[side][5];
+      vn[side][0] = v0;
+      vn[side][1] = v1;
+      B.V[2 * (size_t)kb[side]] = v0;
+      B.V[2 * (size_t)kb[side] + 1] = v1;
     }
   }
   light = wave_sum(light);
