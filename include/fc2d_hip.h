@@ -62,12 +62,22 @@ FC3D_API int fc2d_hip_set_reaction(fc2d_hip_problem* h, const double* reaction);
 FC3D_API int fc2d_hip_get_reaction(fc2d_hip_problem* h, double* reaction);
 FC3D_API int fc2d_hip_get_velocity(fc2d_hip_problem* h, double* velocity);
 
+/* How the sweeps are run (both give the same iterates bit for bit):
+ *   FC2D_SCHEDULE_ONE_WORKGROUP: one launch of one 1024-thread workgroup
+ *     runs the whole solve, colours separated by workgroup barriers;
+ *   FC2D_SCHEDULE_PER_COLOUR: one thread per contact across the GPU, one
+ *     launch per colour, sweeps replayed from a captured graph;
+ *   FC2D_SCHEDULE_AUTO: per colour when some colour holds more than 2048
+ *     contacts, one workgroup otherwise. */
+enum { FC2D_SCHEDULE_AUTO = 0, FC2D_SCHEDULE_ONE_WORKGROUP = 1, FC2D_SCHEDULE_PER_COLOUR = 2 };
+
 typedef struct fc2d_hip_params {
   int max_iter;         /* iparam[SICONOS_IPARAM_MAX_ITER]                    */
   double tol;           /* dparam[SICONOS_DPARAM_TOL]                         */
   int error_evaluation; /* iparam[13]: LIGHT (1) or LIGHT_WITH_FULL_FINAL (2); *
                          * fc2d_nsgs evaluates no other mode (it then sweeps  *
                          * max_iter times and reports info 1)                */
+  int schedule;         /* FC2D_SCHEDULE_*                                    */
 } fc2d_hip_params;
 
 typedef struct fc2d_hip_stats {
@@ -79,10 +89,11 @@ typedef struct fc2d_hip_stats {
   int num_colours;
   int velocity_set;    /* 1 if u = q + W r was computed (full error)         */
   double solve_ms;     /* device time of the solve                           */
+  int schedule;        /* the FC2D_SCHEDULE_* used                            */
 } fc2d_hip_stats;
 
 /* fc2d_nsgs_set_default (fc2d_nsgs.c:785-789) and the registry defaults:
- * max_iter 1000, tol 1e-4, LIGHT_WITH_FULL_FINAL */
+ * max_iter 1000, tol 1e-4, LIGHT_WITH_FULL_FINAL; schedule automatic */
 FC3D_API void fc2d_hip_params_default(fc2d_hip_params* p);
 /* One solve from the resident reaction (fc2d_nsgs.c:297-538, without contact
  * freezing or shuffling).  0 or a NUMERICS_ERR_* code; the reference's info

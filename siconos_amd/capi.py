@@ -108,12 +108,14 @@ class HipStats(C.Structure):
 
 
 class Fc2dParams(C.Structure):
-    _fields_ = [("max_iter", C.c_int), ("tol", C.c_double), ("error_evaluation", C.c_int)]
+    _fields_ = [("max_iter", C.c_int), ("tol", C.c_double), ("error_evaluation", C.c_int),
+                ("schedule", C.c_int)]
 
 
 class Fc2dStats(C.Structure):
     _fields_ = [("info", C.c_int), ("iterations", C.c_int), ("residual", C.c_double),
-                ("num_colours", C.c_int), ("velocity_set", C.c_int), ("solve_ms", C.c_double)]
+                ("num_colours", C.c_int), ("velocity_set", C.c_int), ("solve_ms", C.c_double),
+                ("schedule", C.c_int)]
 
 
 class HipBatchStats(C.Structure):

@@ -1,11 +1,18 @@
 // fc2d_problem.hip — 2-D friction-contact NSGS on gfx950 (fc2d_hip.h).
 //
-// One workgroup of 1024 threads runs a whole solve of fc2d_nsgs
-// (fc2d_nsgs.c:297-538): the colours of the contact graph are swept one
-// after the other, separated by workgroup barriers, each thread taking the
-// colour's contacts in strides; the light error, the LIGHT_WITH_FULL_FINAL
-// tolerance logic and fc2d_compute_error (fc2d_compute_error.c:104-148) run
-// in the same kernel, so a solve is one launch with no host round trip.
+// A solve of fc2d_nsgs (fc2d_nsgs.c:297-538) sweeps the colours of the
+// contact graph one after the other, in one of two schedules:
+//   one workgroup: one launch of 1024 threads runs the whole solve, colours
+//     separated by workgroup barriers, each thread taking the colour's
+//     contacts in strides; the light error, the LIGHT_WITH_FULL_FINAL
+//     tolerance logic and fc2d_compute_error (fc2d_compute_error.c:104-148)
+//     run in the same kernel, so a solve has no host round trip;
+//   per colour: one thread per contact over the whole GPU, one launch per
+//     colour, then a one-workgroup launch for the light error and the
+//     tolerance logic; the sweeps are captured in a graph and replayed
+//     until a device-side stop flag is set.
+// Both run the same per-contact step and the same reductions, so they give
+// the same iterates and decisions bit for bit.
 //
 // Data (colour-major slot order, one slot per contact):
 //   rows: 2 scalar rows per slot in CSR form, entries in the reference's
@@ -121,15 +128,13 @@ __device__ double block_sum(double v, double* sh) {
   return t;
 }
 
-// Entries of a row are loaded CHUNK at a time (columns and values, then the
-// reactions they select) before being added in order, so the loads of a
-// chunk overlap instead of forming one dependent chain per entry.
-constexpr int CHUNK = 8;
-
 // acc += the row's terms in order; own-contact entries (column in the pair
 // of slot s) add w * 0 when own_zero, else nothing.  x: the reactions read
 // (r for the sweep's local problem; own entries never read r there).
-template <bool RESIDUAL>
+// Entries are loaded CHUNK at a time (columns and values, then the reactions
+// they select) before being added in order, so the loads of a chunk overlap
+// instead of forming one dependent chain per entry.
+template <bool RESIDUAL, int CHUNK = 8>
 __device__ __forceinline__ double row_terms(const Fc2dArgs& A, int s, int beg, int end,
                                             double acc) {
   for (int e0 = beg; e0 < end; e0 += CHUNK) {
@@ -156,15 +161,31 @@ __device__ __forceinline__ double row_terms(const Fc2dArgs& A, int s, int beg, i
   return acc;
 }
 
-// q_loc of slot s (fc2d_nsgs.c:97-113 through NM_row_prod_no_diag2)
+// q_loc of slot s (fc2d_nsgs.c:97-113 through NM_row_prod_no_diag2), row by
+// row, C entries in flight at a time.  (Reading both rows' chunks together
+// measured slower: 83 against 61 ms a solve of 7002 contacts.)
+template <int C>
 __device__ __forceinline__ void local_q(const Fc2dArgs& A, int s, double* y) {
   for (int i = 0; i < 2; i++) {
     const int row = 2 * s + i;
     if (A.row_mode == FC2D_ROWS_DOT)
-      y[i] = A.q[row] + row_terms<false>(A, s, A.rp[row], A.rp[row + 1], 0.0);
+      y[i] = A.q[row] + row_terms<false, C>(A, s, A.rp[row], A.rp[row + 1], 0.0);
     else
-      y[i] = row_terms<false>(A, s, A.rp[row], A.rp[row + 1], A.q[row]);
+      y[i] = row_terms<false, C>(A, s, A.rp[row], A.rp[row + 1], A.q[row]);
   }
+}
+
+// The per-contact step of a sweep (fc2d_nsgs.c:448-471): local problem,
+// local solve, light-error term, write-back.
+template <int C>
+__device__ __forceinline__ void sweep_contact(const Fc2dArgs& A, int s) {
+  double ql[2], P[2] = {A.r[2 * s], A.r[2 * s + 1]};
+  local_q<C>(A, s, ql);
+  local_solve(A.diag + 4 * s, A.det[s], ql, A.mu[s], P);
+  const double x0 = A.r[2 * s] - P[0], x1 = A.r[2 * s + 1] - P[1];
+  A.light[s] = x0 * x0 + x1 * x1;
+  A.r[2 * s] = P[0];
+  A.r[2 * s + 1] = P[1];
 }
 
 // fc2d_compute_error with the resident r; writes u.  All threads return it.
@@ -197,56 +218,58 @@ __global__ __launch_bounds__(FC2D_THREADS) void k_fc2d_error(Fc2dArgs A, double*
   if (threadIdx.x == 0) *out = e;
 }
 
+// The light error and the tolerance logic after a sweep (fc2d_nsgs.c:136-142,
+// 191-228), for the light modes.  All threads of the workgroup call it and
+// get the same results.
+__device__ void end_of_sweep(const Fc2dArgs& A, double* sh, int mode, double tol,
+                             double* tolerance, double* error, int* notconv, int* vel) {
+  double l = 0.0, r2 = 0.0;
+  for (int s = threadIdx.x; s < A.nc; s += FC2D_THREADS) {
+    l += A.light[s];
+    r2 += A.r[2 * s] * A.r[2 * s] + A.r[2 * s + 1] * A.r[2 * s + 1];
+  }
+  l = block_sum(l, sh);
+  r2 = block_sum(r2, sh);
+  double err = sqrt(l);
+  const double norm_r = sqrt(r2);
+  if (fabs(norm_r) > DBL_EPSILON) err /= norm_r;
+  *error = err;
+  if (mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT) {
+    *notconv = err < *tolerance ? 0 : 1;
+  } else {
+    *notconv = 1;
+    if (err < *tolerance) {
+      *notconv = 0;
+      const double abs_err = full_error(A, sh);
+      *vel = 1;
+      if (abs_err > tol) {
+        *tolerance = err / abs_err * tol;
+        *notconv = 1;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ bool is_light(int mode) {
+  return mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT ||
+         mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT_WITH_FULL_FINAL;
+}
+
 // fc2d_nsgs.c:448-538 (the branch without freezing) in colour order
 __global__ __launch_bounds__(FC2D_THREADS) void k_fc2d_nsgs(Fc2dArgs A, int itermax, double tol,
                                                            int mode, Fc2dOut* out) {
   __shared__ double sh[FC2D_THREADS / 64];
   int iter = 0, notconv = 1, vel = 0;
   double error = INFINITY, tolerance = tol;
-  const bool light_mode = mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT ||
-                          mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT_WITH_FULL_FINAL;
+  const bool light_mode = is_light(mode);
   while (iter < itermax && notconv) {
     ++iter;
     for (int c = 0; c < A.ncolours; c++) {
-      for (int s = A.cptr[c] + threadIdx.x; s < A.cptr[c + 1]; s += FC2D_THREADS) {
-        double ql[2], P[2] = {A.r[2 * s], A.r[2 * s + 1]};
-        local_q(A, s, ql);
-        local_solve(A.diag + 4 * s, A.det[s], ql, A.mu[s], P);
-        const double x0 = A.r[2 * s] - P[0], x1 = A.r[2 * s + 1] - P[1];
-        A.light[s] = x0 * x0 + x1 * x1;
-        A.r[2 * s] = P[0];
-        A.r[2 * s + 1] = P[1];
-      }
+      for (int s = A.cptr[c] + threadIdx.x; s < A.cptr[c + 1]; s += FC2D_THREADS)
+        sweep_contact<8>(A, s);
       __syncthreads();  // the next colour reads this colour's reactions
     }
-    if (light_mode) {
-      // fc2d_nsgs.c:136-142
-      double l = 0.0, r2 = 0.0;
-      for (int s = threadIdx.x; s < A.nc; s += FC2D_THREADS) {
-        l += A.light[s];
-        r2 += A.r[2 * s] * A.r[2 * s] + A.r[2 * s + 1] * A.r[2 * s + 1];
-      }
-      l = block_sum(l, sh);
-      r2 = block_sum(r2, sh);
-      error = sqrt(l);
-      const double norm_r = sqrt(r2);
-      if (fabs(norm_r) > DBL_EPSILON) error /= norm_r;
-      if (mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT) {
-        notconv = error < tolerance ? 0 : 1;
-      } else {
-        // fc2d_nsgs.c:191-228
-        notconv = 1;
-        if (error < tolerance) {
-          notconv = 0;
-          const double abs_err = full_error(A, sh);
-          vel = 1;
-          if (abs_err > tol) {
-            tolerance = error / abs_err * tol;
-            notconv = 1;
-          }
-        }
-      }
-    }
+    if (light_mode) end_of_sweep(A, sh, mode, tol, &tolerance, &error, &notconv, &vel);
     __syncthreads();
   }
   if (mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT_WITH_FULL_FINAL) {
@@ -256,6 +279,69 @@ __global__ __launch_bounds__(FC2D_THREADS) void k_fc2d_nsgs(Fc2dArgs A, int iter
   }
   if (threadIdx.x == 0) {
     out->iterations = iter;
+    out->notconv = notconv;
+    out->velocity_set = vel;
+    out->error = error;
+  }
+}
+
+// Solve state of the per-colour schedule, in device memory: the loop
+// variables of k_fc2d_nsgs and the solve's parameters, so that one captured
+// graph serves every solve of the problem.
+struct Fc2dState {
+  int iter;
+  int notconv;
+  int vel;
+  int stop;  // no further sweep: converged or iter == itermax
+  int itermax;
+  int mode;
+  double error;
+  double tolerance;
+  double tol;
+};
+
+// One colour of one sweep: one thread per contact of slots [sb, se).
+__global__ __launch_bounds__(256) void k_fc2d_colour(Fc2dArgs A, int sb, int se,
+                                                     const Fc2dState* st) {
+  if (st->stop) return;
+  const int s = sb + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (s < se) sweep_contact<8>(A, s);
+}
+
+// End of a sweep of the per-colour schedule (the loop test and body tail of
+// k_fc2d_nsgs).
+__global__ __launch_bounds__(FC2D_THREADS) void k_fc2d_end_sweep(Fc2dArgs A, Fc2dState* st) {
+  __shared__ double sh[FC2D_THREADS / 64];
+  if (st->stop) return;
+  const int iter = st->iter + 1, mode = st->mode;
+  int notconv = 1, vel = st->vel;
+  double error = st->error, tolerance = st->tolerance;
+  if (is_light(mode)) end_of_sweep(A, sh, mode, st->tol, &tolerance, &error, &notconv, &vel);
+  __syncthreads();  // every thread has read st
+  if (threadIdx.x == 0) {
+    st->iter = iter;
+    st->notconv = notconv;
+    st->vel = vel;
+    st->error = error;
+    st->tolerance = tolerance;
+    st->stop = iter < st->itermax && notconv ? 0 : 1;
+  }
+}
+
+// After the last sweep of the per-colour schedule: LIGHT_WITH_FULL_FINAL's
+// final fc2d_compute_error (fc2d_nsgs.c:525-532), and the results.
+__global__ __launch_bounds__(FC2D_THREADS) void k_fc2d_finish(Fc2dArgs A, const Fc2dState* st,
+                                                             Fc2dOut* out) {
+  __shared__ double sh[FC2D_THREADS / 64];
+  int notconv = st->notconv, vel = st->vel;
+  double error = st->error;
+  if (st->mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT_WITH_FULL_FINAL) {
+    error = full_error(A, sh);
+    vel = 1;
+    notconv = error < st->tol ? 0 : 1;
+  }
+  if (threadIdx.x == 0) {
+    out->iterations = st->iter;
     out->notconv = notconv;
     out->velocity_set = vel;
     out->error = error;
@@ -284,6 +370,9 @@ struct fc2d_hip_problem {
   double *rv = nullptr, *diag = nullptr, *det = nullptr, *q = nullptr, *mu = nullptr;
   double *r = nullptr, *u = nullptr, *light = nullptr, *err = nullptr;
   Fc2dOut* out = nullptr;
+  Fc2dState* st = nullptr;        // per-colour schedule
+  hipGraphExec_t graph = nullptr;  // FC2D_GRAPH_SWEEPS sweeps of it
+  std::vector<int> hcptr;          // colour -> slot range, on the host
   hipEvent_t ev[2] = {nullptr, nullptr};
 
   Fc2dArgs args() const {
@@ -308,6 +397,55 @@ struct fc2d_hip_problem {
   }
 };
 
+// Sweeps per graph replay of the per-colour schedule: the host reads the stop
+// flag after each replay; sweeps after the stop return at once.
+constexpr int FC2D_GRAPH_SWEEPS = 16;
+
+// The schedule a solve uses: the requested one, or automatically one launch
+// per colour when some colour holds more contacts than one workgroup sweeps
+// in two strides (below that, one workgroup's barriers cost less than the
+// launches).  -1 for an unknown request.
+static int fc2d_schedule(const fc2d_hip_problem* h, int requested) {
+  if (requested == FC2D_SCHEDULE_ONE_WORKGROUP || requested == FC2D_SCHEDULE_PER_COLOUR)
+    return requested;
+  if (requested != FC2D_SCHEDULE_AUTO) return -1;
+  int widest = 0;
+  for (size_t c = 0; c + 1 < h->hcptr.size(); c++)
+    widest = std::max(widest, h->hcptr[c + 1] - h->hcptr[c]);
+  return widest > 2 * FC2D_THREADS ? FC2D_SCHEDULE_PER_COLOUR : FC2D_SCHEDULE_ONE_WORKGROUP;
+}
+
+// Captures FC2D_GRAPH_SWEEPS sweeps of the per-colour schedule once per
+// problem; the solve's parameters live in the state, not in the graph.
+static int build_graph(fc2d_hip_problem* h) {
+  if (h->graph) return 0;
+  const Fc2dArgs a = h->args();
+  hipGraph_t g = nullptr;
+  FC3D_HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < FC2D_GRAPH_SWEEPS; k++) {
+    for (int c = 0; c < h->ncolours; c++) {
+      const int sb = h->hcptr[(size_t)c], se = h->hcptr[(size_t)c + 1];
+      if (se <= sb) continue;
+      hipLaunchKernelGGL(k_fc2d_colour, dim3((unsigned)((se - sb + 255) / 256)), dim3(256), 0,
+                         h->stream, a, sb, se, h->st);
+    }
+    hipLaunchKernelGGL(k_fc2d_end_sweep, dim3(1), dim3(FC2D_THREADS), 0, h->stream, a, h->st);
+  }
+  const hipError_t e1 = hipStreamEndCapture(h->stream, &g);
+  if (e1 != hipSuccess) {
+    set_error("fc2d graph capture: %s", hipGetErrorString(e1));
+    return NUMERICS_ERR_DEVICE;
+  }
+  const hipError_t e2 = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e2 != hipSuccess) {
+    h->graph = nullptr;
+    set_error("fc2d graph instantiation: %s", hipGetErrorString(e2));
+    return NUMERICS_ERR_DEVICE;
+  }
+  return 0;
+}
+
 extern "C" {
 
 void fc2d_hip_params_default(fc2d_hip_params* p) {
@@ -315,14 +453,16 @@ void fc2d_hip_params_default(fc2d_hip_params* p) {
   p->max_iter = 1000;
   p->tol = 1e-4;
   p->error_evaluation = SICONOS_NSGS_ERROR_EVALUATION_LIGHT_WITH_FULL_FINAL;
+  p->schedule = FC2D_SCHEDULE_AUTO;
 }
 
 void fc2d_hip_problem_destroy(fc2d_hip_problem* h) {
   if (!h) return;
   if (fc3d::use_device(h->device) == 0) {
     if (h->stream) hipStreamSynchronize(h->stream);
-    void* ptrs[] = {h->cptr, h->rp, h->rc, h->rv, h->diag, h->det, h->q,
-                    h->mu,   h->r,  h->u,  h->light, h->err, h->out};
+    if (h->graph) hipGraphExecDestroy(h->graph);
+    void* ptrs[] = {h->cptr, h->rp, h->rc,    h->rv,  h->diag, h->det, h->q,
+                    h->mu,   h->r,  h->u,     h->light, h->err, h->out, h->st};
     for (void* p : ptrs)
       if (p) hipFree(p);
     for (hipEvent_t e : h->ev)
@@ -467,8 +607,10 @@ int fc2d_hip_problem_create(int nc, const int64_t* row_ptr, const int64_t* cols,
     fc2d_hip_problem_destroy(h);
     return rc;
   }
+  h->hcptr = cptr;
   if (hipMalloc(&h->err, sizeof(double)) != hipSuccess ||
-      hipMalloc(&h->out, sizeof(Fc2dOut)) != hipSuccess) {
+      hipMalloc(&h->out, sizeof(Fc2dOut)) != hipSuccess ||
+      hipMalloc(&h->st, sizeof(Fc2dState)) != hipSuccess) {
     set_error("fc2d_hip_problem_create: allocation failed");
     fc2d_hip_problem_destroy(h);
     return NUMERICS_ERR_DEVICE;
@@ -553,9 +695,44 @@ int fc2d_hip_solve(fc2d_hip_problem* h, const fc2d_hip_params* p, fc2d_hip_stats
         return 0;
       }
   }
-  FC3D_HIP_TRY(hipEventRecord(h->ev[0], h->stream));
-  hipLaunchKernelGGL(k_fc2d_nsgs, dim3(1), dim3(FC2D_THREADS), 0, h->stream, h->args(),
-                     p->max_iter, p->tol, p->error_evaluation, h->out);
+  const int schedule = fc2d_schedule(h, p->schedule);
+  if (schedule < 0) {
+    set_error("fc2d_hip_solve: schedule %d is not 0, 1 or 2", p->schedule);
+    return NUMERICS_ERR_INVALID_ARGUMENT;
+  }
+  s->schedule = schedule;
+  if (schedule == FC2D_SCHEDULE_PER_COLOUR) {
+    Fc2dState init;
+    std::memset(&init, 0, sizeof init);
+    init.notconv = 1;
+    init.stop = p->max_iter > 0 ? 0 : 1;
+    init.itermax = p->max_iter;
+    init.mode = p->error_evaluation;
+    init.error = INFINITY;
+    init.tolerance = p->tol;
+    init.tol = p->tol;
+    FC3D_HIP_TRY(hipMemcpyAsync(h->st, &init, sizeof init, hipMemcpyHostToDevice, h->stream));
+    if ((rc = build_graph(h))) return rc;
+    FC3D_HIP_TRY(hipEventRecord(h->ev[0], h->stream));
+    // every replay advances the sweep counter or finds the stop flag set
+    const long max_replays = (long)p->max_iter / FC2D_GRAPH_SWEEPS + 2;
+    for (long replay = 0;; replay++) {
+      if (replay > max_replays) {
+        set_error("fc2d_hip_solve: sweep loop did not stop after %ld replays", replay);
+        return NUMERICS_ERR_UNKNOWN;
+      }
+      if (init.stop) break;
+      FC3D_HIP_TRY(hipGraphLaunch(h->graph, h->stream));
+      FC3D_HIP_TRY(hipMemcpyAsync(&init, h->st, sizeof init, hipMemcpyDeviceToHost, h->stream));
+      FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    hipLaunchKernelGGL(k_fc2d_finish, dim3(1), dim3(FC2D_THREADS), 0, h->stream, h->args(), h->st,
+                       h->out);
+  } else {
+    FC3D_HIP_TRY(hipEventRecord(h->ev[0], h->stream));
+    hipLaunchKernelGGL(k_fc2d_nsgs, dim3(1), dim3(FC2D_THREADS), 0, h->stream, h->args(),
+                       p->max_iter, p->tol, p->error_evaluation, h->out);
+  }
   FC3D_HIP_TRY(hipGetLastError());
   FC3D_HIP_TRY(hipEventRecord(h->ev[1], h->stream));
   Fc2dOut o;

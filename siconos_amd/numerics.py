@@ -313,8 +313,9 @@ class Fc2dHipProblem:
         self._check(self._lib.fc2d_hip_get_velocity(self._h, dptr(u)), "fc2d_hip_get_velocity")
         return u
 
-    def solve(self, max_iter=1000, tol=1e-4, error_evaluation=2):
-        p = capi.Fc2dParams(max_iter, tol, error_evaluation)
+    def solve(self, max_iter=1000, tol=1e-4, error_evaluation=2, schedule=0):
+        """schedule: 0 automatic, 1 one workgroup, 2 one launch per colour."""
+        p = capi.Fc2dParams(max_iter, tol, error_evaluation, schedule)
         s = capi.Fc2dStats()
         self._check(self._lib.fc2d_hip_solve(self._h, C.byref(p), C.byref(s)), "fc2d_hip_solve")
         return s

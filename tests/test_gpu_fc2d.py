@@ -50,29 +50,64 @@ def test_driver_matches_the_oracle_in_colour_order(lib, name):
     assert res <= 1e-5
 
 
-def test_light_mode_max_iter_and_warm_start(lib):
+@pytest.mark.parametrize("name", FC2D_COLLECTION)
+def test_both_schedules_give_the_same_solve(lib, name):
+    """One workgroup for the whole solve and one launch per colour (replayed
+    from a graph) run the same per-contact step and the same reductions:
+    same reactions, velocities, sweep counts and residual, bit for bit."""
+    p = N.FrictionContactProblem.from_file(fixture_file(name))
+    H = N.Fc2dHipProblem(p)
+    out = {}
+    for sched in (1, 2):
+        H.set_reaction(np.zeros(2 * p.numberOfContacts))
+        s = H.solve(max_iter=10000, tol=1e-5, error_evaluation=2, schedule=sched)
+        assert s.schedule == sched
+        out[sched] = (s.info, s.iterations, s.residual, H.reaction(), H.velocity())
+        print(f"\n[fc2d] {name}: schedule {sched}: {s.iterations} sweeps, {s.solve_ms:.2f} ms")
+    a, b = out[1], out[2]
+    assert a[:3] == b[:3] and a[0] == 0
+    np.testing.assert_array_equal(a[3], b[3])
+    np.testing.assert_array_equal(a[4], b[4])
+
+
+def test_automatic_schedule_and_unknown_schedule(lib):
+    """Colours wider than 2048 contacts take the per-colour schedule."""
+    small = N.Fc2dHipProblem(N.FrictionContactProblem.from_file(
+        fixture_file("FrictionContactProblem00237.dat")))
+    assert small.solve(max_iter=10, schedule=0).schedule == 1
+    with pytest.raises(RuntimeError):
+        small.solve(max_iter=10, schedule=3)
+
+
+@pytest.mark.parametrize("schedule", [1, 2])
+def test_light_mode_max_iter_and_warm_start(lib, schedule):
     path = fixture_file("FrictionContactProblem00237.dat")
     p = N.FrictionContactProblem.from_file(path)
     H = N.Fc2dHipProblem(p)
     order = H.order()
     orc = O.Fc2dProblem(path)
+    # no sweep at all: the final full error of the zero reaction
+    s = H.solve(max_iter=0, tol=1e-5, error_evaluation=2, schedule=schedule)
+    info_o, _, _, it_o, res_o = orc.nsgs(tol=1e-5, max_iter=0, mode=2, order=order)
+    assert s.iterations == it_o == 0 and s.info == info_o
+    assert abs(s.residual - res_o) <= 1e-12 * max(res_o, 1e-300)
     # LIGHT: stops on the light error, no velocity
-    s = H.solve(max_iter=10000, tol=1e-5, error_evaluation=1)
+    s = H.solve(max_iter=10000, tol=1e-5, error_evaluation=1, schedule=schedule)
     info_o, r_o, _, it_o, res_o = orc.nsgs(tol=1e-5, max_iter=10000, mode=1, order=order)
     assert s.info == info_o == 0 and s.iterations == it_o and not s.velocity_set
     np.testing.assert_array_equal(H.reaction(), r_o)
     # FULL is not evaluated by fc2d_nsgs: max_iter sweeps, info 1, residual inf
     H.set_reaction(np.zeros(2 * p.numberOfContacts))
-    s = H.solve(max_iter=7, tol=1e-5, error_evaluation=0)
+    s = H.solve(max_iter=7, tol=1e-5, error_evaluation=0, schedule=schedule)
     assert s.info == 1 and s.iterations == 7 and s.residual == np.inf
     # max_iter bounds the sweeps: info 1
     H.set_reaction(np.zeros(2 * p.numberOfContacts))
-    s = H.solve(max_iter=3, tol=1e-12, error_evaluation=2)
+    s = H.solve(max_iter=3, tol=1e-12, error_evaluation=2, schedule=schedule)
     info_o, r_o, u_o, it_o, res_o = orc.nsgs(tol=1e-12, max_iter=3, mode=2, order=order)
     assert s.info == info_o == 1 and s.iterations == 3
     np.testing.assert_array_equal(H.reaction(), r_o)
     # the resident reaction is the start of the next solve
-    s = H.solve(max_iter=10000, tol=1e-5)
+    s = H.solve(max_iter=10000, tol=1e-5, schedule=schedule)
     info_o, r_o, _, it_o, _ = orc.nsgs(tol=1e-5, max_iter=10000, mode=2, order=order,
                                        reaction=r_o)
     assert s.info == 0 and s.iterations == it_o
