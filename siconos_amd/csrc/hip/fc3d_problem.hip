@@ -236,51 +236,9 @@ __global__ __launch_bounds__(256) void k_residual(SweepPtrs<double> P, const Sol
   }
 }
 
-// Fixed-order sums of NV interleaved partials (part[i * NV + j], j < NV) with
-// one block of 1024 threads: thread t adds entries t, t + 1024, ... in that
-// order (sixteen loads in flight at a time, so a sweep's ~16k slice partials
-// cost one memory round trip), then waves and the 16 wave sums are added in a
-// fixed order.  Results are valid on thread 0.
-template <int NV>
-__device__ void block_sums(const double* __restrict__ part, int n, double* sh, double* out) {
-  double v[NV];
-#pragma unroll
-  for (int j = 0; j < NV; j++) v[j] = 0.0;
-  for (int i = threadIdx.x; i < n; i += 16 * 1024) {
-    double t[16][NV];
-#pragma unroll
-    for (int u = 0; u < 16; u++) {
-      const int k = i + u * 1024;
-#pragma unroll
-      for (int j = 0; j < NV; j++) t[u][j] = k < n ? part[(size_t)k * NV + j] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < 16; u++)
-#pragma unroll
-      for (int j = 0; j < NV; j++)
-        if (i + u * 1024 < n) v[j] += t[u][j];
-  }
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  double w[NV];
-#pragma unroll
-  for (int j = 0; j < NV; j++) w[j] = wave_sum(v[j]);
-  __syncthreads();  // sh may hold an earlier call's sums
-  if (lane == 0)
-#pragma unroll
-    for (int j = 0; j < NV; j++) sh[j * 16 + wave] = w[j];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int j = 0; j < NV; j++) {
-      double t = 0.0;
-      for (int k = 0; k < 16; k++) t += sh[j * 16 + k];
-      out[j] = t;
-    }
-  }
-}
-
 // End of a sweep: light error (fc3d_nsgs.c:90-96) and the convergence test of
-// the chosen evaluation mode (fc3d_nsgs.c:826-863).
+// the chosen evaluation mode (fc3d_nsgs.c:826-863).  In the velocity form the
+// tail launch does the same work instead, when the sweep has a tail.
 __global__ __launch_bounds__(1024) void k_finalize_sweep(const DevParams* prm, SolveState* st,
                                                          const double* part, int nslices,
                                                          int want_phase) {
@@ -288,42 +246,7 @@ __global__ __launch_bounds__(1024) void k_finalize_sweep(const DevParams* prm, S
   if (st->stop || st->phase != want_phase) return;
   double sums[2];
   block_sums<2>(part, nslices, sh, sums);
-  const double light = sums[0], nr2 = sums[1];
-  if (threadIdx.x != 0) return;
-  st->iter += 1;
-  st->nf = sqrt(light);
-  double incr = sqrt(light);
-  const double nr = sqrt(nr2);
-  if (fabs(nr) > DBL_EPSILON) incr /= nr;
-  st->incr = incr;
-  if (want_phase == 1) {
-    // FP32 phase: hand over to FP64 sweeps once the FP32 light error is small
-    st->fp32_iter += 1;
-    st->need_full = 0;
-    if (incr < prm->fp32_tol || st->iter >= prm->max_iter) st->stop = 1;
-    return;
-  }
-  const int mode = prm->mode;
-  if (mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT) {
-    st->notconv = incr < st->working_tol ? 0 : 1;
-    st->need_full = 0;
-  } else if (mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT_WITH_FULL_FINAL) {
-    if (incr >= st->working_tol) {
-      st->notconv = 1;
-      st->need_full = 0;
-    } else {
-      st->need_full = 1;
-    }
-  } else {
-    const int f = st->freq;
-    st->need_full = f > 0 ? (st->iter % f == 0) : 1;
-    if (!st->need_full) {
-      st->full = 1e24;
-      st->incr = 1e24;
-      st->notconv = 1;
-    }
-  }
-  if (!st->need_full && (st->notconv == 0 || st->iter >= prm->max_iter)) st->stop = 1;
+  if (threadIdx.x == 0) end_sweep_state(prm, st, sums[0], sums[1], want_phase);
 }
 
 // fc3d_compute_error.c:100-115 and, unless `plain`, the tolerance manager
@@ -768,22 +691,22 @@ static void launch_sweep_vel_tail(fc3d_hip_problem* h, int phase, int c0, hipEve
     SweepPtrs<float> P = ptrs32(h);
     if (h->accel_on) P.f = h->aa32.f;
     if (e0)
-      hipExtLaunchKernelGGL(k, g, b, 0, h->stream, e0, e1, 0, P, bodies32(h), h->prm,
-                            (const SolveState*)h->st, (const int*)h->d_cps, c0, h->ncolours, 1);
+      hipExtLaunchKernelGGL(k, g, b, 0, h->stream, e0, e1, 0, P, bodies32(h), h->prm, h->st,
+                            (const int*)h->d_cps, c0, h->ncolours, 1, 1, h->nslices);
     else
       hipLaunchKernelGGL(k, g, b, 0, h->stream, P, bodies32(h), h->prm, h->st, h->d_cps, c0,
-                         h->ncolours, 1);
+                         h->ncolours, 1, 1, h->nslices);
   } else {
     auto k = h->frames ? k_sweep_vel_tail<double, true> : k_sweep_vel_tail<double, false>;
     const dim3 b(vel_tail_threads<double>());
     SweepPtrs<double> P = ptrs64(h, h->part);
     if (h->accel_on) P.f = h->aa64.f;
     if (e0)
-      hipExtLaunchKernelGGL(k, g, b, 0, h->stream, e0, e1, 0, P, bodies64(h), h->prm,
-                            (const SolveState*)h->st, (const int*)h->d_cps, c0, h->ncolours, 0);
+      hipExtLaunchKernelGGL(k, g, b, 0, h->stream, e0, e1, 0, P, bodies64(h), h->prm, h->st,
+                            (const int*)h->d_cps, c0, h->ncolours, 0, 1, h->nslices);
     else
       hipLaunchKernelGGL(k, g, b, 0, h->stream, P, bodies64(h), h->prm, h->st, h->d_cps, c0,
-                         h->ncolours, 0);
+                         h->ncolours, 0, 1, h->nslices);
   }
 }
 
@@ -871,6 +794,7 @@ static void enqueue_sweep(fc3d_hip_problem* h, int phase, int schedule, hipEvent
       launch_sweep_vel(h, phase, sb, se, t ? ev[2 * c] : nullptr, t ? ev[2 * c + 1] : nullptr);
     }
     if (ct < h->ncolours) {
+      // the tail launch also ends the sweep (k_finalize_sweep's work)
       const bool t = ev && 2 * ct + 1 < nev;
       launch_sweep_vel_tail(h, phase, ct, t ? ev[2 * ct] : nullptr, t ? ev[2 * ct + 1] : nullptr);
     }
@@ -886,8 +810,9 @@ static void enqueue_sweep(fc3d_hip_problem* h, int phase, int schedule, hipEvent
         launch_sweep(h, phase, schedule, grid, sb, se);
     }
   }
-  hipLaunchKernelGGL(k_finalize_sweep, dim3(1), dim3(1024), 0, h->stream, h->prm, h->st, h->part,
-                     h->nslices, phase);
+  if (!h->vel_on || vel_tail_begin(h, phase) >= h->ncolours)
+    hipLaunchKernelGGL(k_finalize_sweep, dim3(1), dim3(1024), 0, h->stream, h->prm, h->st,
+                       h->part, h->nslices, phase);
   if (!phase) {
     hipLaunchKernelGGL(k_residual, dim3((h->nslices + 3) / 4), dim3(256), 0, h->stream,
                        ptrs64(h, h->part2), h->st, h->nslices, 0, nullptr, 0);

@@ -108,6 +108,96 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Fixed-order sums of NV interleaved partials (part[i * NV + j], j < NV) as
+// 1024 threads add them: thread t adds entries t, t + 1024, ... in that order
+// (sixteen loads in flight at a time, so a sweep's ~16k slice partials cost
+// one memory round trip), each wave of 64 threads adds its values with
+// wave_sum, then the 16 wave sums are added in order.  A workgroup of R
+// threads (R divides 1024, R >= 64) computes the same bits: thread r stands
+// for threads r, r + R, ..., and its wave for the matching waves.  sh holds
+// NV * 16 doubles; the results are valid on thread 0.
+template <int NV, int R = 1024>
+__device__ void block_sums(const double* __restrict__ part, int n, double* sh, double* out) {
+  static_assert(R >= 64 && R <= 1024 && 1024 % R == 0, "R must divide 1024");
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();  // sh may hold an earlier call's sums
+#pragma unroll
+  for (int m = 0; m < 1024 / R; m++) {
+    double v[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) v[j] = 0.0;
+    for (int i = (int)threadIdx.x + m * R; i < n; i += 16 * 1024) {
+      double t[16][NV];
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int k = i + u * 1024;
+#pragma unroll
+        for (int j = 0; j < NV; j++) t[u][j] = k < n ? part[(size_t)k * NV + j] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 16; u++)
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+          if (i + u * 1024 < n) v[j] += t[u][j];
+    }
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+      const double w = wave_sum(v[j]);
+      if (lane == 0) sh[j * 16 + wave + m * (R / 64)] = w;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+      double t = 0.0;
+      for (int k = 0; k < 16; k++) t += sh[j * 16 + k];
+      out[j] = t;
+    }
+  }
+}
+
+// State update at the end of a sweep from its light-error sums (one thread):
+// fc3d_nsgs.c:90-96 and the convergence test of the chosen evaluation mode
+// (fc3d_nsgs.c:826-863).
+__device__ inline void end_sweep_state(const DevParams* prm, SolveState* st, double light,
+                                       double nr2, int want_phase) {
+  st->iter += 1;
+  st->nf = sqrt(light);
+  double incr = sqrt(light);
+  const double nr = sqrt(nr2);
+  if (fabs(nr) > DBL_EPSILON) incr /= nr;
+  st->incr = incr;
+  if (want_phase == 1) {
+    // FP32 phase: hand over to FP64 sweeps once the FP32 light error is small
+    st->fp32_iter += 1;
+    st->need_full = 0;
+    if (incr < prm->fp32_tol || st->iter >= prm->max_iter) st->stop = 1;
+    return;
+  }
+  const int mode = prm->mode;
+  if (mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT) {
+    st->notconv = incr < st->working_tol ? 0 : 1;
+    st->need_full = 0;
+  } else if (mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT_WITH_FULL_FINAL) {
+    if (incr >= st->working_tol) {
+      st->notconv = 1;
+      st->need_full = 0;
+    } else {
+      st->need_full = 1;
+    }
+  } else {
+    const int f = st->freq;
+    st->need_full = f > 0 ? (st->iter % f == 0) : 1;
+    if (!st->need_full) {
+      st->full = 1e24;
+      st->incr = 1e24;
+      st->notconv = 1;
+    }
+  }
+  if (!st->need_full && (st->notconv == 0 || st->iter >= prm->max_iter)) st->stop = 1;
+}
+
 // y += sum_k W_k r_{col(k)} over the off-diagonal blocks of the lane's row, in
 // the reference row order (SBM_row_prod_no_diag_3x3, SparseBlockMatrix.c:
 // 1431-1490).  The slice index is wave-uniform, so its metadata are scalar

@@ -340,10 +340,15 @@ constexpr int vel_tail_threads() {
 }
 
 template <typename T, bool FRAMES>
+//
+// With finalize, the same workgroup then ends the sweep as k_finalize_sweep
+// does (the sums of all nslices slice partials, with the same bits, and the
+// state update), which saves that launch.
 __global__ __launch_bounds__(vel_tail_threads<T>()) void k_sweep_vel_tail(
-    SweepPtrs<T> P, BodyPtrs<T> B, const DevParams* __restrict__ prm,
-    const SolveState* __restrict__ st, const int* __restrict__ colour_slices, int c_begin,
-    int c_end, int want_phase) {
+    SweepPtrs<T> P, BodyPtrs<T> B, const DevParams* __restrict__ prm, SolveState* st,
+    const int* __restrict__ colour_slices, int c_begin, int c_end, int want_phase,
+    int finalize, int nslices) {
+  __shared__ double sh[2 * 16];
   if (st->stop || st->phase != want_phase) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   constexpr int waves = vel_tail_threads<T>() / 64;
@@ -353,6 +358,10 @@ __global__ __launch_bounds__(vel_tail_threads<T>()) void k_sweep_vel_tail(
       vel_slice<T, false, FRAMES>(P, B, prm, st, s, lane, want_phase);
     __syncthreads();  // the next colour reads this colour's body velocities
   }
+  if (!finalize) return;
+  double sums[2];
+  block_sums<2, vel_tail_threads<T>()>(P.part, nslices, sh, sums);
+  if (threadIdx.x == 0) end_sweep_state(prm, st, sums[0], sums[1], want_phase);
 }
 
 }  // namespace fc3d

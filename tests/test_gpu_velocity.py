@@ -150,7 +150,9 @@ def test_driver_maps_the_fp32_local_solver(lib):
 def test_tail_launch_gives_the_same_iterates(lib, precision):
     """The last small colours run in one workgroup (k_sweep_vel_tail) instead
     of one launch each: same per-slice computation, so the same iterates and
-    sweep counts bit for bit (FC3D_HIP_VEL_TAIL=0 turns the tail off)."""
+    sweep counts bit for bit (FC3D_HIP_VEL_TAIL=0 turns the tail off).  The
+    tail launch also ends the sweep with 512 or 256 threads standing for
+    k_finalize_sweep's 1024: the same light errors bit for bit."""
     import os
     g = N.generate_granular_pile(20000, 50000, 0.1, 0.9, 5)
     H = _with_bodies(g)
@@ -164,6 +166,7 @@ def test_tail_launch_gives_the_same_iterates(lib, precision):
     finally:
         del os.environ["FC3D_HIP_VEL_TAIL"]
     assert a.iterations == b.iterations and a.info == b.info == 0
+    assert a.incr_error == b.incr_error and a.residual == b.residual
     np.testing.assert_array_equal(H.reaction(), ra)
     # instrumented mode times the tail as one launch
     i = H.solve(N.hip_params(tol=1e-300, max_iter=5, warm_start=1, velocity_form=1,
