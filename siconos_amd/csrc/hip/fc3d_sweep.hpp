@@ -119,32 +119,42 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <int NV, int R = 1024>
 __device__ void block_sums(const double* __restrict__ part, int n, double* sh, double* out) {
   static_assert(R >= 64 && R <= 1024 && 1024 % R == 0, "R must divide 1024");
+  constexpr int M = 1024 / R;          // threads each real thread stands for
+  constexpr int MB = M < 2 ? M : 2;    // of those, loaded together
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   __syncthreads();  // sh may hold an earlier call's sums
 #pragma unroll
-  for (int m = 0; m < 1024 / R; m++) {
-    double v[NV];
+  for (int m0 = 0; m0 < M; m0 += MB) {
+    double v[MB][NV];
 #pragma unroll
-    for (int j = 0; j < NV; j++) v[j] = 0.0;
-    for (int i = (int)threadIdx.x + m * R; i < n; i += 16 * 1024) {
-      double t[16][NV];
+    for (int mm = 0; mm < MB; mm++)
 #pragma unroll
-      for (int u = 0; u < 16; u++) {
-        const int k = i + u * 1024;
+      for (int j = 0; j < NV; j++) v[mm][j] = 0.0;
+    for (int base = 0; base < n; base += 16 * 1024) {
+      double t[MB][16][NV];
 #pragma unroll
-        for (int j = 0; j < NV; j++) t[u][j] = k < n ? part[(size_t)k * NV + j] : 0.0;
+      for (int mm = 0; mm < MB; mm++)
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+          const int k = base + (int)threadIdx.x + (m0 + mm) * R + u * 1024;
+#pragma unroll
+          for (int j = 0; j < NV; j++) t[mm][u][j] = k < n ? part[(size_t)k * NV + j] : 0.0;
+        }
+#pragma unroll
+      for (int mm = 0; mm < MB; mm++)
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+#pragma unroll
+          for (int j = 0; j < NV; j++)
+            if (base + (int)threadIdx.x + (m0 + mm) * R + u * 1024 < n) v[mm][j] += t[mm][u][j];
+    }
+#pragma unroll
+    for (int mm = 0; mm < MB; mm++)
+#pragma unroll
+      for (int j = 0; j < NV; j++) {
+        const double w = wave_sum(v[mm][j]);
+        if (lane == 0) sh[j * 16 + wave + (m0 + mm) * (R / 64)] = w;
       }
-#pragma unroll
-      for (int u = 0; u < 16; u++)
-#pragma unroll
-        for (int j = 0; j < NV; j++)
-          if (i + u * 1024 < n) v[j] += t[u][j];
-    }
-#pragma unroll
-    for (int j = 0; j < NV; j++) {
-      const double w = wave_sum(v[j]);
-      if (lane == 0) sh[j * 16 + wave + m * (R / 64)] = w;
-    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -159,43 +169,48 @@ __device__ void block_sums(const double* __restrict__ part, int n, double* sh, d
 
 // State update at the end of a sweep from its light-error sums (one thread):
 // fc3d_nsgs.c:90-96 and the convergence test of the chosen evaluation mode
-// (fc3d_nsgs.c:826-863).
+// (fc3d_nsgs.c:826-863).  The state and the parameters it reads are loaded
+// at once into registers and the state stored back whole, so the update
+// waits on one memory round trip instead of one per field.
 __device__ inline void end_sweep_state(const DevParams* prm, SolveState* st, double light,
                                        double nr2, int want_phase) {
-  st->iter += 1;
-  st->nf = sqrt(light);
+  SolveState x = *st;
+  const double fp32_tol = prm->fp32_tol;
+  const int max_iter = prm->max_iter, mode = prm->mode;
+  x.iter += 1;
+  x.nf = sqrt(light);
   double incr = sqrt(light);
   const double nr = sqrt(nr2);
   if (fabs(nr) > DBL_EPSILON) incr /= nr;
-  st->incr = incr;
+  x.incr = incr;
   if (want_phase == 1) {
     // FP32 phase: hand over to FP64 sweeps once the FP32 light error is small
-    st->fp32_iter += 1;
-    st->need_full = 0;
-    if (incr < prm->fp32_tol || st->iter >= prm->max_iter) st->stop = 1;
-    return;
-  }
-  const int mode = prm->mode;
-  if (mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT) {
-    st->notconv = incr < st->working_tol ? 0 : 1;
-    st->need_full = 0;
-  } else if (mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT_WITH_FULL_FINAL) {
-    if (incr >= st->working_tol) {
-      st->notconv = 1;
-      st->need_full = 0;
-    } else {
-      st->need_full = 1;
-    }
+    x.fp32_iter += 1;
+    x.need_full = 0;
+    if (incr < fp32_tol || x.iter >= max_iter) x.stop = 1;
   } else {
-    const int f = st->freq;
-    st->need_full = f > 0 ? (st->iter % f == 0) : 1;
-    if (!st->need_full) {
-      st->full = 1e24;
-      st->incr = 1e24;
-      st->notconv = 1;
+    if (mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT) {
+      x.notconv = incr < x.working_tol ? 0 : 1;
+      x.need_full = 0;
+    } else if (mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT_WITH_FULL_FINAL) {
+      if (incr >= x.working_tol) {
+        x.notconv = 1;
+        x.need_full = 0;
+      } else {
+        x.need_full = 1;
+      }
+    } else {
+      const int f = x.freq;
+      x.need_full = f > 0 ? (x.iter % f == 0) : 1;
+      if (!x.need_full) {
+        x.full = 1e24;
+        x.incr = 1e24;
+        x.notconv = 1;
+      }
     }
+    if (!x.need_full && (x.notconv == 0 || x.iter >= max_iter)) x.stop = 1;
   }
-  if (!st->need_full && (st->notconv == 0 || st->iter >= prm->max_iter)) st->stop = 1;
+  *st = x;
 }
 
 // y += sum_k W_k r_{col(k)} over the off-diagonal blocks of the lane's row, in
