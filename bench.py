@@ -12,7 +12,7 @@ beside it on the same input: the plain FP64 coloured NSGS (reference
 options, rows of W, zero start), the fast method on the rows of W, with
 safeguarded Anderson acceleration, and from the learned warm start.
 
-Prints one JSON line (rank 0).  `--gpus N` with N > 1 is refused: this build
+Prints one JSON line (rank 0).  `--gpus N` with N > 1 is refused: the library
 runs on one GPU.
 """
 import argparse
@@ -230,7 +230,7 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+        print(json.dumps({"error": "the library runs on one GPU; --gpus must be 1 "
                                    f"(got {args.gpus})"}))
         sys.exit(2)
 

@@ -15,7 +15,7 @@ extern "C" {
 #endif
 
 /* FrictionContact_options.h:60-73 lists the reference fc2d ids (400-411; 409
- * and 410 are its CUDA sweeps).  This build registers one more: */
+ * and 410 are its CUDA sweeps).  This library registers one more: */
 enum { SICONOS_FRICTION_2D_NSGS_HIP = 440 };
 
 /* fc2d_driver.c:38-150: validation, registry lookup, init / solve / free.

@@ -500,7 +500,8 @@ FC3D_API int fc3d_hip_assemble(int nbodies, int nc, const int* body_a, const int
 /* ------------------------------------------------------------------------ */
 FC3D_API int fc3d_colour_contacts(int nc, const int64_t* row_ptr, const int64_t* col_idx,
                                   int* colour);
-/* Checker: 1 if no block W_ij (i != j) joins two contacts of one colour. */
+/* Validity of a colouring: 1 if no block W_ij (i != j) joins two contacts of
+ * one colour. */
 FC3D_API int fc3d_colouring_is_valid(int nc, const int64_t* row_ptr, const int64_t* col_idx,
                                      const int* colour);
 

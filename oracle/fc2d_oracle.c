@@ -1,7 +1,7 @@
 /* fc2d_oracle.c — CPU oracle for the fc2d NSGS path.
  *
  * TEST INFRASTRUCTURE ONLY: a plain-C restatement of the reference's 2-D
- * friction-contact NSGS, used by tests/ as the checker.  The product library
+ * friction-contact NSGS, which tests/ compare the GPU path against.  The product library
  * never links, loads or calls it.  Each function cites the reference
  * file:line it follows (paths relative to the Siconos tree).
  *

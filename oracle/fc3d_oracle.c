@@ -1,7 +1,8 @@
 /* fc3d_oracle.c — CPU oracle for the fc3d NSGS hot path (TEST INFRASTRUCTURE).
  *
  * Plain C99 restatement of the reference algorithm.  Only tests/, smoke() and
- * bench.py's cpu_baseline leg use it, as the checker; the product never does.
+ * bench.py's cpu_baseline leg use it, as the reference results are compared
+ * against; the product never does.
  * Built with -ffp-contract=off so every product of the reference is rounded
  * before the sum it feeds, as in an IEEE-754 build of the reference C.
  *

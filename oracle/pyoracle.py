@@ -1,7 +1,8 @@
 """ctypes loader of the CPU oracle (test infrastructure only).
 
 Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
-checker; the product (siconos_amd) never imports this module.
+reference results are compared against; the product (siconos_amd) never
+imports this module.
 """
 import ctypes as C
 import os

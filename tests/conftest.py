@@ -1,5 +1,5 @@
 """Shared fixtures.  GPU tests carry @pytest.mark.gpu and call the HIP path
-through the C-ABI; the CPU oracle (oracle/) is the checker."""
+through the C-ABI; results are compared against the CPU oracle (oracle/)."""
 import os
 import sys
 
