@@ -288,8 +288,10 @@ enum SICONOS_FRICTION_3D_NSGS_HIP_DPARAM {
   SICONOS_FRICTION_3D_NSGS_HIP_FP32_TOL = 23,    /* residual at which FP32 hands over   */
   SICONOS_FRICTION_3D_NSGS_HIP_TIME_SOLVE = 24,  /* out: device time of sweeps, ms      */
   SICONOS_FRICTION_3D_NSGS_HIP_ACCEL_REG = 25,   /* Anderson regularisation (x trace)    */
-  SICONOS_FRICTION_3D_NSGS_HIP_FP32_LOCAL_SOLVER = 26 /* local solver id of the FP32 sweeps,
-                                                        0 = the internal solver's          */
+  SICONOS_FRICTION_3D_NSGS_HIP_FP32_LOCAL_SOLVER = 26, /* local solver id of the FP32 sweeps,
+                                                         0 = the internal solver's         */
+  SICONOS_FRICTION_3D_NSGS_HIP_FP32_LOCAL_TOL = 27     /* local tolerance of the FP32 sweeps,
+                                                         0 = the default (1e-6)             */
 };
 
 /* ------------------------------------------------------------------------ */

@@ -3,8 +3,8 @@
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
  * algorithm, used by tests/, __graft_entry__.smoke() and the cpu_baseline leg
  * of bench.py, as the reference results are compared against.  The product
- * library never links, loads or
- * calls it.  Each function cites the reference file:line it follows
+ * library never links, loads or calls it.  Each function cites the reference
+ * file:line it follows
  * (paths relative to the Siconos tree).
  *
  * Pinning (what checks each part against something other than itself) is

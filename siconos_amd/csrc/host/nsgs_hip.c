@@ -63,6 +63,8 @@ int fc3d_nsgs_hip_params_from_options(const SolverOptions* o, fc3d_hip_params* p
   if (o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_ACCEL_REG] > 0.0)
     p->accel_reg = o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_ACCEL_REG];
   p->fp32_local_solver = (int)o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_FP32_LOCAL_SOLVER];
+  if (o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_FP32_LOCAL_TOL] > 0.0)
+    p->fp32_local_tol = o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_FP32_LOCAL_TOL];
   return 0;
 }
 
@@ -139,6 +141,7 @@ void fc3d_nsgs_hip_set_default(SolverOptions* o) {
   o->iparam[SICONOS_FRICTION_3D_NSGS_HIP_ACCEL_EVERY] = 2;
   o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_ACCEL_REG] = 1e-10;
   o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_FP32_LOCAL_SOLVER] = 0.0;
+  o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_FP32_LOCAL_TOL] = 0.0;
 }
 
 static int nsgs_hip_solve_wrap(void* problem, double* reaction, double* velocity,

@@ -141,6 +141,15 @@ def test_driver_maps_the_fp32_local_solver(lib):
     u = np.zeros(3 * g.nc)
     assert N.fc3d_driver(prob, r, u, o) == 0
     assert o.dparam[N.SICONOS_DPARAM_RESIDU] <= 1e-6
+    # dparam[27]: the FP32 sweeps' local tolerance reaches the solver (other
+    # FP32 iterates), and the FP64 tolerance still holds
+    r_default = r.copy()
+    o.dparam[27] = 1e-7
+    r[:] = 0.0
+    assert N.fc3d_driver(prob, r, u, o) == 0
+    assert o.dparam[N.SICONOS_DPARAM_RESIDU] <= 1e-6
+    assert not np.array_equal(r, r_default)
+    o.dparam[27] = 0.0
     o.dparam[26] = 123.0
     r[:] = 0.0
     assert N.fc3d_driver(prob, r, u, o) != 0
