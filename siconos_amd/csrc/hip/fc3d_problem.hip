@@ -43,13 +43,27 @@ __global__ __launch_bounds__(256, MINW) void k_sweep(SweepPtrs<T> P,
                                                      const DevParams* __restrict__ prm,
                                                      const SolveState* __restrict__ st,
                                                      int s_begin, int s_end, int want_phase) {
-  if (st->stop || st->phase != want_phase) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int s = __builtin_amdgcn_readfirstlane(s_begin + xcd_block() * 4 + wave);
   if (s >= s_end) return;
+  // The slot's own loads, the slice's plane range and the step constants are
+  // issued before the stop flag is tested, so the flag's latency overlaps
+  // theirs (padding slots read their in-bounds planes too); the empty asm
+  // keeps the compiler from sinking them below the test.
+  const int stop = st->stop, phase = st->phase;
+  const StepConsts sc = step_consts<T>(prm);
   const T tol = T(want_phase ? prm->fp32_local_tol : st->local_tol);
+  const int len = P.len[s * 64 + lane];
+  SlotInputs<T> in;
+  load_slot<T>(P, s, lane, &in);
+  int koff, kmax;
+  slice_planes<T>(P, s, &koff, &kmax);
+  step_consts_ready(sc);
+  asm volatile("" ::"v"(len), "v"(in.lp.mu), "v"(in.ro.x), "v"(in.pli), "v"(in.lp.W[8]),
+               "v"(in.ql[2]), "s"(tol), "s"(koff), "s"(kmax), "s"(stop), "s"(phase));
+  if (stop || phase != want_phase) return;
   double light, nr2;
-  sweep_slot<T, U, PF>(P, prm, s, lane, tol, &light, &nr2);
+  sweep_slot_loaded<T, U, PF>(P, sc, s, lane, tol, len, in, koff, kmax, &light, &nr2);
   light = wave_sum(light);
   nr2 = wave_sum(nr2);
   if (lane == 0) {

@@ -220,10 +220,8 @@ __device__ inline void end_sweep_state(const DevParams* prm, SolveState* st, dou
 // valid column) so that U blocks and U neighbour reactions are in flight
 // together; only blocks k < len enter the sum.
 template <typename T, int U, int PF>
-__device__ __forceinline__ void gather_row(const SweepPtrs<T>& P, int s, int lane, int len,
-                                           T* y) {
-  const int koff = __builtin_amdgcn_readfirstlane(P.koff[s]);
-  const int kmax = __builtin_amdgcn_readfirstlane(P.kmax[s]);
+__device__ __forceinline__ void gather_row(const SweepPtrs<T>& P, int s, int lane, int len, T* y,
+                                           int koff, int kmax) {
   const int* __restrict__ colp = P.col + (size_t)koff * 64 + lane;
   const T* __restrict__ valp = P.vals + (size_t)koff * 9 * 64 + lane;
   int k0 = 0;
@@ -418,6 +416,28 @@ __device__ __forceinline__ void finish_slot(const SweepPtrs<T>& P, const DevPara
 // slice s: local problem (fc3d_nsgs_update), local solve, relaxation, light
 // error term, optional filter, write-back.  Returns |r_old - r_new|^2 and the
 // squared norm of the accepted reaction (0 for padding slots).
+// The slice's plane range (wave-uniform scalar loads).
+template <typename T>
+__device__ __forceinline__ void slice_planes(const SweepPtrs<T>& P, int s, int* koff, int* kmax) {
+  *koff = __builtin_amdgcn_readfirstlane(P.koff[s]);
+  *kmax = __builtin_amdgcn_readfirstlane(P.kmax[s]);
+}
+
+// sweep_slot with the slot's inputs, the slice's plane range and the step
+// constants already loaded by the caller.
+template <typename T, int U, int PF>
+__device__ __forceinline__ void sweep_slot_loaded(const SweepPtrs<T>& P, const StepConsts& c,
+                                                  int s, int lane, T tol, int len,
+                                                  SlotInputs<T>& in, int koff, int kmax,
+                                                  double* light, double* nr2) {
+  *light = 0.0;
+  *nr2 = 0.0;
+  if (len >= 0) {
+    gather_row<T, U, PF>(P, s, lane, len, in.ql, koff, kmax);
+    finish_slot<T>(P, c, s, lane, tol, in, light, nr2);
+  }
+}
+
 template <typename T, int U, int PF>
 __device__ __forceinline__ void sweep_slot(const SweepPtrs<T>& P, const DevParams* __restrict__ prm,
                                            int s, int lane, T tol, double* light, double* nr2) {
@@ -428,7 +448,9 @@ __device__ __forceinline__ void sweep_slot(const SweepPtrs<T>& P, const DevParam
     // local-solve inputs first: their loads overlap the gather
     SlotInputs<T> in;
     load_slot<T>(P, s, lane, &in);
-    gather_row<T, U, PF>(P, s, lane, len, in.ql);
+    int koff, kmax;
+    slice_planes<T>(P, s, &koff, &kmax);
+    gather_row<T, U, PF>(P, s, lane, len, in.ql, koff, kmax);
     finish_slot<T>(P, prm, s, lane, tol, in, light, nr2);
   }
 }
