@@ -164,23 +164,24 @@ __device__ __forceinline__ double row_terms(const Fc2dArgs& A, int s, int beg, i
 // q_loc of slot s (fc2d_nsgs.c:97-113 through NM_row_prod_no_diag2), row by
 // row, C entries in flight at a time.  (Reading both rows' chunks together
 // measured slower: 83 against 61 ms a solve of 7002 contacts.)
+// rp: the slot's row pointers rp[2 s], rp[2 s + 1], rp[2 s + 2].
 template <int C>
-__device__ __forceinline__ void local_q(const Fc2dArgs& A, int s, double* y) {
+__device__ __forceinline__ void local_q(const Fc2dArgs& A, int s, const int* rp, double* y) {
   for (int i = 0; i < 2; i++) {
     const int row = 2 * s + i;
     if (A.row_mode == FC2D_ROWS_DOT)
-      y[i] = A.q[row] + row_terms<false, C>(A, s, A.rp[row], A.rp[row + 1], 0.0);
+      y[i] = A.q[row] + row_terms<false, C>(A, s, rp[i], rp[i + 1], 0.0);
     else
-      y[i] = row_terms<false, C>(A, s, A.rp[row], A.rp[row + 1], A.q[row]);
+      y[i] = row_terms<false, C>(A, s, rp[i], rp[i + 1], A.q[row]);
   }
 }
 
 // The per-contact step of a sweep (fc2d_nsgs.c:448-471): local problem,
 // local solve, light-error term, write-back.
 template <int C>
-__device__ __forceinline__ void sweep_contact(const Fc2dArgs& A, int s) {
+__device__ __forceinline__ void sweep_contact(const Fc2dArgs& A, int s, const int* rp) {
   double ql[2], P[2] = {A.r[2 * s], A.r[2 * s + 1]};
-  local_q<C>(A, s, ql);
+  local_q<C>(A, s, rp, ql);
   local_solve(A.diag + 4 * s, A.det[s], ql, A.mu[s], P);
   const double x0 = A.r[2 * s] - P[0], x1 = A.r[2 * s + 1] - P[1];
   A.light[s] = x0 * x0 + x1 * x1;
@@ -265,8 +266,10 @@ __global__ __launch_bounds__(FC2D_THREADS) void k_fc2d_nsgs(Fc2dArgs A, int iter
   while (iter < itermax && notconv) {
     ++iter;
     for (int c = 0; c < A.ncolours; c++) {
-      for (int s = A.cptr[c] + threadIdx.x; s < A.cptr[c + 1]; s += FC2D_THREADS)
-        sweep_contact<8>(A, s);
+      for (int s = A.cptr[c] + threadIdx.x; s < A.cptr[c + 1]; s += FC2D_THREADS) {
+        const int rp[3] = {A.rp[2 * s], A.rp[2 * s + 1], A.rp[2 * s + 2]};
+        sweep_contact<8>(A, s, rp);
+      }
       __syncthreads();  // the next colour reads this colour's reactions
     }
     if (light_mode) end_of_sweep(A, sh, mode, tol, &tolerance, &error, &notconv, &vel);
@@ -303,9 +306,15 @@ struct Fc2dState {
 // One colour of one sweep: one thread per contact of slots [sb, se).
 __global__ __launch_bounds__(256) void k_fc2d_colour(Fc2dArgs A, int sb, int se,
                                                      const Fc2dState* st) {
-  if (st->stop) return;
+  // the contact's row pointers are loaded before the stop flag is tested, so
+  // the two latencies overlap; the empty asm keeps the loads above the test
   const int s = sb + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (s < se) sweep_contact<8>(A, s);
+  const int t = s < se ? s : sb;
+  const int rp[3] = {A.rp[2 * t], A.rp[2 * t + 1], A.rp[2 * t + 2]};
+  const int stop = st->stop;
+  asm volatile("" ::"v"(rp[0]), "v"(rp[1]), "v"(rp[2]), "s"(stop));
+  if (stop) return;
+  if (s < se) sweep_contact<8>(A, s, rp);
 }
 
 // End of a sweep of the per-colour schedule (the loop test and body tail of
