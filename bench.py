@@ -45,7 +45,7 @@ WORKLOADS = {
 # WRITE x1.000).  A profiler pass cannot run inside the timed bench, so the
 # figure is carried here with its source.
 PMC_TRAFFIC = {
-    "pile1m": {"bytes_per_launch": 44.808e6, "fetch": 36.658e6, "write": 8.150e6,
+    "pile1m": {"bytes_per_launch": 44.783e6, "fetch": 36.662e6, "write": 8.121e6,
                "source": "profiles/k_sweep_vel_pile1m_hbm_traffic.md"},
 }
 
