@@ -1213,6 +1213,9 @@ static bool rigid_frames(int nc, const int* body_a, const int* body_b, const dou
                          const std::vector<int>& orig2slot, int nslices, std::vector<double>* Fs) {
   Fs->assign((size_t)nslices * 15 * 64, 0.0);
   for (int c = 0; c < nc; c++) {
+    // the sweep rebuilds W_cc side by side, without the cross terms of a
+    // contact whose two sides are the same body
+    if (body_a[c] >= 0 && body_a[c] == body_b[c]) return false;
     const double* hc = H + 36 * (size_t)c;
     const bool has[2] = {body_a[c] >= 0, body_b[c] >= 0};
     double f[15] = {0};

@@ -235,3 +235,74 @@ def test_rigid_body_blocks_use_the_frames_layout_and_others_do_not(oracle):
             info, ro, uo, it, res = oracle.nsgs(WW, qq, mm, tol=1e-300, max_iter=K,
                                                 order=P.order())
             assert np.linalg.norm(rv - ro) <= 1e-10 * np.linalg.norm(ro), (kind, K)
+
+
+def _dense_problem(nb, ba, bb, H, minv, vfree):
+    """W = J M^-1 J^T and q = J v_free assembled densely on the host, J the
+    contact Jacobian with the 3x6 blocks of H (a contact with the same body
+    on both sides adds both blocks to that body's columns)."""
+    nc = len(ba)
+    J = np.zeros((3 * nc, 6 * nb))
+    for c in range(nc):
+        for side, k in enumerate((ba[c], bb[c])):
+            if k >= 0:
+                J[3 * c:3 * c + 3, 6 * k:6 * k + 6] += H[36 * c + 18 * side:
+                                                        36 * c + 18 * side + 18].reshape(3, 6, order="F")
+    W = J @ np.diag(minv) @ J.T
+    return N.SparseBlock3.from_dense(W), J @ vfree, W
+
+
+def test_same_body_and_one_body_contacts(oracle):
+    """Contacts with one body (side b = -1) and with the same body on both
+    sides: the velocity form adds side b's update to side a's in registers
+    (FP32) or reads it back (FP64); both must track the oracle's iterates."""
+    rng = np.random.default_rng(21)
+    nb, nc = 150, 450
+    ba = rng.integers(0, nb, nc).astype(np.int32)
+    bb = rng.integers(0, nb, nc).astype(np.int32)
+    bb[:40] = -1          # one body
+    bb[40:70] = ba[40:70]  # the same body on both sides
+    same = bb == ba
+    bb[70:][same[70:]] = (ba[70:][same[70:]] + 1) % nb
+    H = rng.normal(size=36 * nc)
+    minv = rng.uniform(0.5, 2.0, size=6 * nb)
+    vfree = rng.normal(size=6 * nb)
+    mu = rng.uniform(0.1, 0.9, size=nc)
+    Wsb, q, _ = _dense_problem(nb, ba, bb, H, minv, vfree)
+    P = N.HipProblem(Wsb, q, mu)
+    P.set_bodies(nb, ba, bb, H, minv, vfree)
+    for K in (3, 15):
+        info, ro, uo, it, res = oracle.nsgs(Wsb, q, mu, tol=1e-300, max_iter=K, order=P.order())
+        P.solve(N.hip_params(tol=1e-300, max_iter=K, warm_start=1, velocity_form=1))
+        assert np.linalg.norm(P.reaction() - ro) <= 1e-10 * np.linalg.norm(ro), K
+        P.solve(N.hip_params(tol=1e-300, max_iter=K, warm_start=1, velocity_form=1, precision=1,
+                             fp32_tol=1e-300))
+        assert np.linalg.norm(P.reaction() - ro) <= 1e-4 * np.linalg.norm(ro), K
+    s = P.solve(N.hip_params(tol=1e-6, max_iter=5000, warm_start=1, velocity_form=1, precision=1,
+                             fp32_tol=1e-6, fp32_local_solver=550))
+    assert s.info == 0 and s.residual <= 1e-6
+    assert abs(P.compute_error() - s.residual) <= 1e-12 + 1e-9 * s.residual
+
+
+def test_frames_layout_refuses_a_same_body_contact(oracle):
+    """The frames layout rebuilds W_cc side by side; a contact with the same
+    body on both sides needs the cross terms, so such a problem keeps the
+    general layout even with FC3D_HIP_FRAMES=1, and stays exact."""
+    import os
+    g = N.generate_granular_pile(300, 800, 0.1, 0.9, 4)
+    ba = np.array(g.body_a, dtype=np.int32)
+    bb = np.array(g.body_b, dtype=np.int32)
+    c = int(np.nonzero((ba >= 0) & (bb >= 0))[0][0])
+    bb[c] = ba[c]
+    Wsb, q, _ = _dense_problem(g.nbodies, ba, bb, np.asarray(g.H), np.asarray(g.minv),
+                               np.asarray(g.vfree))
+    os.environ["FC3D_HIP_FRAMES"] = "1"
+    try:
+        P = N.HipProblem(Wsb, q, g.mu)
+        P.set_bodies(g.nbodies, ba, bb, g.H, g.minv, g.vfree)
+    finally:
+        del os.environ["FC3D_HIP_FRAMES"]
+    assert P.body_layout() == 1
+    P.solve(N.hip_params(tol=1e-300, max_iter=10, warm_start=1, velocity_form=1))
+    info, ro, uo, it, res = oracle.nsgs(Wsb, q, g.mu, tol=1e-300, max_iter=10, order=P.order())
+    assert np.linalg.norm(P.reaction() - ro) <= 1e-10 * np.linalg.norm(ro)
