@@ -237,18 +237,25 @@ def test_rigid_body_blocks_use_the_frames_layout_and_others_do_not(oracle):
             assert np.linalg.norm(rv - ro) <= 1e-10 * np.linalg.norm(ro), (kind, K)
 
 
-def _dense_problem(nb, ba, bb, H, minv, vfree):
+def _dense_problem(nb, ba, bb, H, minv, vfree, minv_diagonal=True):
     """W = J M^-1 J^T and q = J v_free assembled densely on the host, J the
     contact Jacobian with the 3x6 blocks of H (a contact with the same body
-    on both sides adds both blocks to that body's columns)."""
+    on both sides adds both blocks to that body's columns); M^-1 is given per
+    body as 6 diagonal entries or as a 6x6 column-major block."""
     nc = len(ba)
+    if minv_diagonal:
+        Mi = np.diag(minv)
+    else:
+        Mi = np.zeros((6 * nb, 6 * nb))
+        for k in range(nb):
+            Mi[6 * k:6 * k + 6, 6 * k:6 * k + 6] = minv[36 * k:36 * k + 36].reshape(6, 6, order="F")
     J = np.zeros((3 * nc, 6 * nb))
     for c in range(nc):
         for side, k in enumerate((ba[c], bb[c])):
             if k >= 0:
                 J[3 * c:3 * c + 3, 6 * k:6 * k + 6] += H[36 * c + 18 * side:
                                                         36 * c + 18 * side + 18].reshape(3, 6, order="F")
-    W = J @ np.diag(minv) @ J.T
+    W = J @ Mi @ J.T
     return N.SparseBlock3.from_dense(W), J @ vfree, W
 
 
@@ -306,3 +313,34 @@ def test_frames_layout_refuses_a_same_body_contact(oracle):
     P.solve(N.hip_params(tol=1e-300, max_iter=10, warm_start=1, velocity_form=1))
     info, ro, uo, it, res = oracle.nsgs(Wsb, q, g.mu, tol=1e-300, max_iter=10, order=P.order())
     assert np.linalg.norm(P.reaction() - ro) <= 1e-10 * np.linalg.norm(ro)
+
+
+def test_full_mass_blocks(oracle):
+    """Bodies with full 6x6 M^-1 blocks (minv_diagonal = 0): the update reads
+    the block after the local solve instead of six diagonal entries; the
+    iterates track the oracle's in FP64 and in FP32."""
+    rng = np.random.default_rng(23)
+    nb, nc = 120, 360
+    ba = rng.integers(0, nb, nc).astype(np.int32)
+    bb = rng.integers(0, nb, nc).astype(np.int32)
+    bb[:30] = -1
+    bb[30:50] = ba[30:50]
+    same = bb == ba
+    bb[50:][same[50:]] = (ba[50:][same[50:]] + 1) % nb
+    H = rng.normal(size=36 * nc)
+    minv = np.empty(36 * nb)
+    for k in range(nb):
+        A = rng.normal(size=(6, 6))
+        minv[36 * k:36 * k + 36] = (A @ A.T + 0.5 * np.eye(6)).reshape(-1, order="F")
+    vfree = rng.normal(size=6 * nb)
+    mu = rng.uniform(0.1, 0.9, size=nc)
+    Wsb, q, _ = _dense_problem(nb, ba, bb, H, minv, vfree, minv_diagonal=False)
+    P = N.HipProblem(Wsb, q, mu)
+    P.set_bodies(nb, ba, bb, H, minv, vfree, minv_diagonal=False)
+    for K in (3, 15):
+        info, ro, uo, it, res = oracle.nsgs(Wsb, q, mu, tol=1e-300, max_iter=K, order=P.order())
+        P.solve(N.hip_params(tol=1e-300, max_iter=K, warm_start=1, velocity_form=1))
+        assert np.linalg.norm(P.reaction() - ro) <= 1e-10 * np.linalg.norm(ro), K
+        P.solve(N.hip_params(tol=1e-300, max_iter=K, warm_start=1, velocity_form=1, precision=1,
+                             fp32_tol=1e-300))
+        assert np.linalg.norm(P.reaction() - ro) <= 1e-4 * np.linalg.norm(ro), K
