@@ -123,7 +123,8 @@ def batch_leg(steps, warmup, cpu):
            "contacts": nc}
     for name, prm in (("fp64", N.hip_params(tol=1e-6, max_iter=1000)),
                       ("fp32_fp64", N.hip_params(tol=1e-6, max_iter=1000, precision=1,
-                                                 fp32_tol=1e-6, fp32_local_solver=550))):
+                                                 fp32_tol=1e-6, fp32_local_solver=550,
+                                                 fp32_local_tol=1e-7))):
         zero = np.zeros(3 * nc)
         for _ in range(warmup):
             B.set_reaction(zero)
