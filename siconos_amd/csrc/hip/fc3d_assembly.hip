@@ -49,6 +49,32 @@ __global__ void k_asm_g(int nc, const int* __restrict__ ba, const int* __restric
   }
 }
 
+// G_{c,sc} H_{d,sd}^T for one pair of sides, 3x3 column-major.
+__device__ inline void side_product(const double* __restrict__ g, const double* __restrict__ h,
+                                    double (&s)[9]) {
+  double gl[18], hl[18];
+#pragma unroll
+  for (int e = 0; e < 18; e++) {
+    gl[e] = g[e];
+    hl[e] = h[e];
+  }
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      double t = 0.0;
+#pragma unroll
+      for (int l = 0; l < 6; l++) t += gl[i + 3 * l] * hl[j + 3 * l];
+      s[i + 3 * j] = t;
+    }
+}
+
+// One thread per stored block W_cd: the sum over the pairs of sides (sc, sd)
+// that name the same body, in the order (0,0), (0,1), (1,0), (1,1). A block
+// off the diagonal usually has one such pair, so the first pair found is
+// computed by every lane at its own addresses, without divergence; the
+// others (the diagonal's second side, two contacts between the same two
+// bodies, contacts with a body on both sides) follow in a loop.
 __global__ void k_asm_blocks(int64_t nnzb, const int* __restrict__ row, const int* __restrict__ col,
                              const int* __restrict__ ba, const int* __restrict__ bb,
                              const double* __restrict__ H, const double* __restrict__ G,
@@ -57,31 +83,22 @@ __global__ void k_asm_blocks(int64_t nnzb, const int* __restrict__ row, const in
   if (b >= nnzb) return;
   const int c = row[b], d = col[b];
   const int kc[2] = {ba[c], bb[c]}, kd[2] = {ba[d], bb[d]};
+  unsigned m = 0;
+#pragma unroll
+  for (int sc = 0; sc < 2; sc++)
+#pragma unroll
+    for (int sd = 0; sd < 2; sd++)
+      if (kc[sc] >= 0 && kd[sd] == kc[sc]) m |= 1u << (2 * sc + sd);
   double acc[9];
 #pragma unroll
   for (int e = 0; e < 9; e++) acc[e] = 0.0;
-  for (int sc = 0; sc < 2; sc++) {
-    if (kc[sc] < 0) continue;
-    for (int sd = 0; sd < 2; sd++) {
-      if (kd[sd] != kc[sc]) continue;
-      const double* g = G + 36 * (size_t)c + 18 * sc;
-      const double* h = H + 36 * (size_t)d + 18 * sd;
-      double gl[18], hl[18];
+  while (m) {
+    const int p = __builtin_ctz(m), sc = p >> 1, sd = p & 1;
+    m &= m - 1;
+    double s[9];
+    side_product(G + 36 * (size_t)c + 18 * sc, H + 36 * (size_t)d + 18 * sd, s);
 #pragma unroll
-      for (int e = 0; e < 18; e++) {
-        gl[e] = g[e];
-        hl[e] = h[e];
-      }
-#pragma unroll
-      for (int j = 0; j < 3; j++)
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-          double s = 0.0;
-#pragma unroll
-          for (int l = 0; l < 6; l++) s += gl[i + 3 * l] * hl[j + 3 * l];
-          acc[i + 3 * j] += s;
-        }
-    }
+    for (int e = 0; e < 9; e++) acc[e] += s[e];
   }
 #pragma unroll
   for (int e = 0; e < 9; e++) W[9 * b + e] = acc[e];
