@@ -82,3 +82,36 @@ def test_assembly_refuses_bad_body_indices(lib):
     H = np.zeros(36)
     with pytest.raises(RuntimeError, match="-2"):
         N.assemble(2, [0], [5], H, np.ones(12), np.zeros(12))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("diagonal", [True, False])
+def test_assembly_of_blocks_with_several_shared_sides(lib, diagonal):
+    """Blocks that sum more than one pair of sides: two contacts between the
+    same two bodies (in either order), and contacts with one body on both
+    sides (four pairs on their diagonal block), against the dense product."""
+    rng = np.random.default_rng(21)
+    nb, nc = 12, 60
+    ba = rng.integers(0, nb, nc).astype(np.int32)
+    bb = ((ba + 1 + rng.integers(0, nb - 1, nc)) % nb).astype(np.int32)
+    ba[40:50], bb[40:50] = ba[0:10], bb[0:10]      # the same pair, same order
+    ba[50:55], bb[50:55] = bb[10:15], ba[10:15]    # the same pair, swapped
+    bb[55:60] = ba[55:60]                          # one body on both sides
+    bb[30:33] = -1
+    H = rng.normal(size=36 * nc)
+    if diagonal:
+        minv = rng.uniform(0.5, 2.0, size=6 * nb)
+    else:
+        Ms = []
+        for _ in range(nb):
+            A = rng.normal(size=(6, 6))
+            Ms.append((A @ A.T + 6 * np.eye(6)).ravel(order="F"))
+        minv = np.concatenate(Ms)
+    vfree = rng.normal(size=6 * nb)
+    W, q, _ = N.assemble(nb, ba, bb, H, minv, vfree, minv_diagonal=diagonal)
+    Wd, qd = dense_from_bodies(nb, ba, bb, H, minv, vfree, diagonal)
+    np.testing.assert_allclose(W.to_dense(), Wd, rtol=0, atol=1e-12 * np.abs(Wd).max())
+    np.testing.assert_allclose(q, qd, rtol=0, atol=1e-12 * np.abs(qd).max())
+    # the case mix the test is for
+    pairs = lambda c, d: sum(1 for x in (ba[c], bb[c]) for y in (ba[d], bb[d]) if x >= 0 and x == y)
+    assert pairs(0, 40) == 2 and pairs(10, 50) == 2 and pairs(55, 55) == 4
