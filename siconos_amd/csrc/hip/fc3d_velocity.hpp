@@ -185,7 +185,7 @@ __device__ __forceinline__ void vel_slice(const SweepPtrs<T>& P, const BodyPtrs<
   const int len = P.len[slot];
   SlotInputs<T> in;
   // FP64: the streamed planes are loaded non-temporal, which keeps V and M^-1
-  // in L2 (5% a sweep); in FP32 it costs 2%
+  // in L2 (5% a sweep); in FP32 it costs 9%
   constexpr bool NT = sizeof(T) == 8;
   if (FRAMES)
     load_slot_state<T>(P, s, lane, &in);  // W_cc is recomputed below
