@@ -474,6 +474,7 @@ struct fc3d_hip_problem {
   std::vector<int> slot2orig;         // -1 for padding
   std::vector<int> orig2slot;
   std::vector<double> colour_bytes64, colour_bytes32;  // algorithmic bytes per colour pass
+  std::vector<double> colour_nc;                       // contacts per colour
   // device
   int *len = nullptr, *dpos = nullptr, *koff = nullptr, *kmax = nullptr, *col = nullptr;
   double *vals = nullptr, *diag = nullptr, *q = nullptr, *mu = nullptr, *pli = nullptr;
@@ -496,6 +497,9 @@ struct fc3d_hip_problem {
   double* plibak = nullptr;
   // Anderson acceleration (allocated on first use)
   int accel_on = 0;
+  // the FP32 sweeps' local solver keeps no PLI state (550, 551, 552): their
+  // kernels neither load nor store it, 1% a solve on the 1M pile
+  int pli32_off = 0, graph_pli32_off[2] = {0, 0};
   int aa_nblocks = 0;
   AccelBufs<double> aa64 = {};
   AccelBufs<float> aa32 = {};
@@ -550,7 +554,7 @@ static SweepPtrs<float> ptrs32(fc3d_hip_problem* h) {
   SweepPtrs<float> P;
   P.len = h->len; P.dpos = h->dpos; P.koff = h->koff; P.kmax = h->kmax; P.col = h->col;
   P.vals = h->vals32; P.diag = h->diag32; P.q = h->q32; P.mu = h->mu32; P.r = h->r32;
-  P.u = nullptr; P.pli = h->pli32; P.part = h->part;
+  P.u = nullptr; P.pli = h->pli32_off ? nullptr : h->pli32; P.part = h->part;
   return P;
 }
 
@@ -783,7 +787,19 @@ static int timed_launches(const fc3d_hip_problem* h, int phase, int schedule) {
   }
   return schedule == FC3D_SCHEDULE_DATAFLOW ? 1 : h->ncolours;
 }
+static double timed_bytes_all(const fc3d_hip_problem* h, int phase, int schedule, int c);
+// Without the PLI state, an FP32 pass moves 8 bytes a contact less.
 static double timed_bytes(const fc3d_hip_problem* h, int phase, int schedule, int c) {
+  double t = timed_bytes_all(h, phase, schedule, c);
+  if (!phase || !h->pli32_off) return t;
+  const size_t nc = h->colour_nc.size();
+  size_t end = (size_t)c + 1;
+  if (h->vel_on ? c >= vel_tail_begin(h, phase) : schedule == FC3D_SCHEDULE_DATAFLOW) end = nc;
+  for (size_t k = h->vel_on || schedule != FC3D_SCHEDULE_DATAFLOW ? (size_t)c : 0; k < end; k++)
+    t -= 8.0 * h->colour_nc[k];
+  return t;
+}
+static double timed_bytes_all(const fc3d_hip_problem* h, int phase, int schedule, int c) {
   if (h->vel_on) {
     const std::vector<double>& b = phase ? h->vel_bytes32 : h->vel_bytes64;
     if (c < vel_tail_begin(h, phase)) return b[(size_t)c];
@@ -1022,7 +1038,7 @@ static int reset_accel(fc3d_hip_problem* h) {
 static int build_graph(fc3d_hip_problem* h, int phase, int sweeps, int schedule) {
   if (h->graph[phase] && h->graph_sweeps[phase] == sweeps && h->graph_schedule[phase] == schedule &&
       h->graph_accel[phase] == h->accel_on && h->graph_vel[phase] == h->vel_on &&
-      h->graph_tail[phase] == vel_tail_begin(h, phase))
+      h->graph_tail[phase] == vel_tail_begin(h, phase) && h->graph_pli32_off[phase] == h->pli32_off)
     return 0;
   if (h->graph[phase]) {
     hipGraphExecDestroy(h->graph[phase]);
@@ -1039,6 +1055,7 @@ static int build_graph(fc3d_hip_problem* h, int phase, int sweeps, int schedule)
   h->graph_accel[phase] = h->accel_on;
   h->graph_vel[phase] = h->vel_on;
   h->graph_tail[phase] = vel_tail_begin(h, phase);
+  h->graph_pli32_off[phase] = h->pli32_off;
   return 0;
 }
 
@@ -1393,6 +1410,8 @@ int fc3d_hip_problem_create(int nc, const int64_t* row_ptr, const int64_t* col_i
   h->orig2slot = L.orig2slot;
   h->colour_bytes64 = L.colour_bytes64;
   h->colour_bytes32 = L.colour_bytes32;
+  h->colour_nc.assign(h->colour_slice_ptr.size() - 1, 0.0);
+  for (int c : h->colour) h->colour_nc[(size_t)c] += 1.0;
   build_dataflow(L, h->colour.data(), &h->dfh);
   if ((rc = dev_upload(&h->len, L.len)) || (rc = dev_upload(&h->dpos, L.dpos)) ||
       (rc = dev_upload(&h->koff, L.koff)) || (rc = dev_upload(&h->kmax, L.kmax)) ||
@@ -1857,6 +1876,10 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
   const int mixed = p->precision == 1;
   if (mixed && (rc = ensure_fp32(h))) return rc;
 
+  {
+    const int eff32 = ls32 ? ls32 : ls;
+    h->pli32_off = eff32 == 550 || eff32 == 551 || eff32 == 552;
+  }
   h->accel_on = 0;  // the autotuner times plain matrix-form sweeps
   h->vel_on = 0;
   int sched[2] = {p->schedule, p->schedule};

@@ -60,7 +60,8 @@ struct SweepPtrs {
   const T* mu;      // [slot]
   typename vec4<T>::type* r;
   typename vec4<T>::type* u;
-  T* pli;           // per-slot PLI step (persistent local state)
+  T* pli;           // per-slot PLI step (persistent local state); null when the
+                    // local solver keeps none (550, 551, 552): not loaded, not stored
   double* part;     // per-slice partial sums
   typename vec4<T>::type* f = nullptr;  // if set: r_new - r_old of the sweep
 };
@@ -316,7 +317,7 @@ __device__ __forceinline__ void load_slot(const SweepPtrs<T>& P, int s, int lane
   for (int e = 0; e < 9; e++) in->lp.W[e] = ld<NT>(&P.diag[(size_t)(s * 9 + e) * 64 + lane]);
   in->lp.mu = P.mu[slot];
   in->ro = P.r[slot];
-  in->pli = P.pli[slot];
+  in->pli = P.pli ? P.pli[slot] : T(0);
 #pragma unroll
   for (int e = 0; e < 3; e++) in->ql[e] = P.q[(size_t)(s * 3 + e) * 64 + lane];
 }
@@ -329,7 +330,7 @@ __device__ __forceinline__ void load_slot_state(const SweepPtrs<T>& P, int s, in
   const int slot = s * 64 + lane;
   in->lp.mu = P.mu[slot];
   in->ro = P.r[slot];
-  in->pli = P.pli[slot];
+  in->pli = P.pli ? P.pli[slot] : T(0);
 }
 
 // The constants of the per-contact step, read from DevParams once per wave
@@ -392,7 +393,7 @@ __device__ __forceinline__ void finish_slot(const SweepPtrs<T>& P, const StepCon
   typename vec4<T>::type out = ro;
   if (accept) { out.x = lr[0]; out.y = lr[1]; out.z = lr[2]; }
   P.r[slot] = out;
-  P.pli[slot] = pli;
+  if (P.pli) P.pli[slot] = pli;
   if (out_r) *out_r = out;
   if (P.f) {
     typename vec4<T>::type d = out;
