@@ -43,7 +43,9 @@ WORKLOADS = {
 # WRITE_SIZE) over `bench.py --steps 1 --warmup 0 --no-cpu --no-batch`,
 # scaled by the readings of a plain 1 GiB device copy (FETCH x2.000,
 # WRITE x1.000).  A profiler pass cannot run inside the timed bench, so the
-# figure is carried here with its source.
+# figure is carried here with its source.  It was measured before the FP32
+# sweeps stopped loading and storing the PLI state for local solver 550,
+# which saves 8 bytes a contact (about 0.7 MB of a big colour's launch).
 PMC_TRAFFIC = {
     "pile1m": {"bytes_per_launch": 44.783e6, "fetch": 36.662e6, "write": 8.121e6,
                "source": "profiles/k_sweep_vel_pile1m_hbm_traffic.md"},
