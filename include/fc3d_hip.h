@@ -79,6 +79,11 @@ typedef struct fc3d_hip_params {
    * FP64 sweeps always use local_solver, and the result is judged on the
    * FP64 residual. */
   int fp32_local_solver;
+  /* Contact freezing (iparam[SICONOS_NSGS_FREEZING_CONTACT], fc3d_nsgs.c:
+   * 719-800): a contact whose step meets the reference's criteria from the
+   * 10th sweep on is skipped for this many sweeps; 0: off.  FP64 sweeps on
+   * the rows of W only. */
+  int freeze_contacts;
 } fc3d_hip_params;
 
 typedef struct fc3d_hip_stats {
@@ -165,6 +170,17 @@ FC3D_API int fc3d_hip_get_velocity(const fc3d_hip_problem* h, double* velocity);
 
 /* fc3d_nsgs.c:560-917 in colour order on the resident reaction. */
 FC3D_API int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats* s);
+/* fc3d_hip_solve with an observer (no reference counterpart: it carries the
+ * reference's per-sweep statistics callback, SolverOptions.h:36-44 called at
+ * fc3d_nsgs.c:103-110).  The observer runs on the host after every graph
+ * replay, i.e. every sweeps_per_launch sweeps (1: after every sweep, as the
+ * reference does), with the sweeps done so far and the last light error;
+ * while it runs, fc3d_hip_get_reaction and fc3d_hip_get_velocity return the
+ * current iterate and the velocity of the last full residual. */
+typedef void (*fc3d_hip_sweep_observer)(void* env, int sweeps_done, double incr_error);
+FC3D_API int fc3d_hip_solve_observed(fc3d_hip_problem* h, const fc3d_hip_params* p,
+                                     fc3d_hip_stats* s, fc3d_hip_sweep_observer observer,
+                                     void* env);
 /* fc3d_compute_error.c:70-122 on the resident reaction; writes the resident
  * velocity.  norm < 0 uses the stored |q|. */
 FC3D_API int fc3d_hip_compute_error(fc3d_hip_problem* h, double norm, double* error);

@@ -290,8 +290,16 @@ enum SICONOS_FRICTION_3D_NSGS_HIP_DPARAM {
   SICONOS_FRICTION_3D_NSGS_HIP_ACCEL_REG = 25,   /* Anderson regularisation (x trace)    */
   SICONOS_FRICTION_3D_NSGS_HIP_FP32_LOCAL_SOLVER = 26, /* local solver id of the FP32 sweeps,
                                                          0 = the internal solver's         */
-  SICONOS_FRICTION_3D_NSGS_HIP_FP32_LOCAL_TOL = 27     /* local tolerance of the FP32 sweeps,
+  SICONOS_FRICTION_3D_NSGS_HIP_FP32_LOCAL_TOL = 27,    /* local tolerance of the FP32 sweeps,
                                                          0 = the default (1e-6)             */
+  /* Device residency across fc3d_driver / fc3d_compute_error calls on the same
+   * NumericsMatrix: 0 (default) keeps the device problem (colouring, layout,
+   * tuned schedule, graphs) while the block structure is unchanged and
+   * re-uploads W's values, q and mu on every call; 1 also skips W's values
+   * while the storage's NumericsDataVersion is unchanged and non-zero
+   * (NumericsMatrix.c:112-135; the caller bumps it when values change);
+   * -1 builds and frees a device problem per call. */
+  SICONOS_FRICTION_3D_NSGS_HIP_RESIDENCY = 28
 };
 
 /* ------------------------------------------------------------------------ */
@@ -438,6 +446,14 @@ FC3D_API int fc3d_compute_error(FrictionContactProblem* problem, double* z, doub
 FC3D_API void fc3d_nsgs_hip(FrictionContactProblem* problem, double* reaction, double* velocity,
                             int* info, SolverOptions* options);
 FC3D_API void fc3d_nsgs_hip_set_default(SolverOptions* options);
+/* Device problems FC3D_NSGS_HIP and fc3d_compute_error keep resident between
+ * calls (dparam[SICONOS_FRICTION_3D_NSGS_HIP_RESIDENCY]; no reference
+ * counterpart: the reference keeps no device copy).
+ * clear drops those of M (NULL: all) that no call is using; NM_free drops
+ * those of the matrix it frees.  stats: calls that reused a resident problem,
+ * device problems built, and reuses that re-uploaded W's values. */
+FC3D_API void fc3d_hip_resident_clear(const NumericsMatrix* M);
+FC3D_API void fc3d_hip_resident_stats(int64_t* hits, int64_t* creates, int64_t* value_uploads);
 
 /* ------------------------------------------------------------------------ */
 /* Seeded problem generators (no reference counterpart: the reference gets W

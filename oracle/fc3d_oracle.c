@@ -18,16 +18,29 @@
  *                              row of the reference fixture ACinputs.dat, plus
  *                              finite differences; mutation 1 must fail.
  *   rho split spectral norm  : closed form for W = diag(2,7,7) blocks.
+ *   eig_3x3, spectral rho    : numpy.linalg.eigvalsh on random symmetric and
+ *                              SPD matrices (repeated eigenvalues included),
+ *                              the unsorted diagonal case; mutation 9 must
+ *                              fail.
  *   fc3d_compute_error       : zero at the FC3D_Example1 closed-form solution;
  *                              independent numpy evaluation elsewhere;
  *                              mutations 2 and 3 must fail.
  *   local solvers            : closed forms for W = diag(a,b,b) (take-off,
  *                              stick, slide) and a brute-force 1-D angle search
  *                              on FrictionContact3D_1c.dat.
+ *   OC_PROJ step (552)       : a brute-force solution is a fixed point of the
+ *                              step; from zero with W = diag(a,b,b) the step
+ *                              is the Moreau projection of -(q_n + mu|q_t|,
+ *                              q_t)/a; mutations 7 and 8 must fail.
  *   fc3d_nsgs                : FC3D_Example1 closed form; residual <= tol on the
  *                              reference fixtures (data_collection_1/3 files)
  *                              under the test_nsgs_1.c option sets; sweep-order
  *                              independence inside a colour.
+ *   contact freezing         : the reference's freeze collection converges
+ *                              (test_nsgs_freeze_1.c); a Python replay of the
+ *                              counters, criteria, reset rule and light error
+ *                              from the reactions after every sweep (through
+ *                              the callback); mutation 10 must fail.
  *   greedy colouring         : brute-force validity + minimality check
  *                              (each contact's colour is the smallest free one).
  * Unpinned (no fixture or import can pin them here): ULP-level agreement with
@@ -356,6 +369,51 @@ void oracle_rho_split_spectral_norm_cond(const double* W, double* rho) {
   rho[0] = 1.0 / W[0];
   rho[1] = 2.0 * (sw - dw) / ((sw + dw) * (sw + dw));
   rho[2] = rho[1];
+}
+
+/* op3x3.h:1065-1111: eigenvalues of a symmetric column-major 3x3 matrix by
+ * the trigonometric method.  A diagonal matrix returns its diagonal as stored
+ * (unsorted); otherwise eig[0] >= eig[1] >= eig[2]. */
+void oracle_eig_3x3(const double* a, double* eig) {
+  const double a00 = a[0], a10 = a[1], a20 = a[2], a01 = a[3], a11 = a[4], a21 = a[5];
+  const double a02 = a[6], a12 = a[7], a22 = a[8];
+  const double pi = M_PI;
+  double p1 = a01 * a01 + a02 * a02 + a12 * a12;
+  if (p1 == 0) {
+    eig[0] = a00;
+    eig[1] = a11;
+    eig[2] = a22;
+    return;
+  }
+  double q = (a00 + a11 + a22) / 3.0;
+  double p2 = (a00 - q) * (a00 - q) + (a11 - q) * (a11 - q) + (a22 - q) * (a22 - q) + 2 * p1;
+  double p = sqrt(p2 / 6.0);
+  double b00 = (1 / p) * (a00 - q), b11 = (1 / p) * (a11 - q), b22 = (1 / p) * (a22 - q);
+  double b01 = (1 / p) * a01, b02 = (1 / p) * a02, b10 = (1 / p) * a10;
+  double b12 = (1 / p) * a12, b20 = (1 / p) * a20, b21 = (1 / p) * a21;
+  double det = b00 * b11 * b22 + b01 * b12 * b20 + b02 * b10 * b21 - b00 * b12 * b21 -
+               b01 * b10 * b22 - b02 * b11 * b20;
+  double r = det / 2.0;
+  double phi;
+  if (r <= -1)
+    phi = pi / 3.0;
+  else if (r >= 1)
+    phi = 0.0;
+  else
+    phi = acos(r) / 3;
+  if (g_mut == ORACLE_MUT_EIG_PHASE) phi = -phi;
+  eig[0] = q + 2 * p * cos(phi);
+  eig[2] = q + 2 * p * cos(phi + (2 * pi / 3));
+  eig[1] = 3 * q - eig[0] - eig[2];
+}
+
+/* fc3d_AlartCurnier_functions.c:814-829 */
+void oracle_rho_spectral_norm(const double* W, double* rho) {
+  double eig[3];
+  oracle_eig_3x3(W, eig);
+  rho[0] = 1.0 / eig[0];
+  rho[1] = rho[0];
+  rho[2] = rho[0];
 }
 
 /* ------------------------------------------------------------------------ */
@@ -709,10 +767,11 @@ static int pli_solve(const local_problem* lp, double* r, SolverOptions* o, doubl
 
 /* fc3d_projection.c:564-603 */
 static int proj_solve(const local_problem* lp, double* r) {
-  double an = 1. / lp->W[0];
+  double an = 1. / lp->W[g_mut == ORACLE_MUT_PROJ_RHO ? 4 : 0];
   double w[3];
   local_velocity_inline(lp, r, w);
   double nUT = sqrt(w[1] * w[1] + w[2] * w[2]);
+  if (g_mut == ORACLE_MUT_PROJ_DROP_MU) nUT = 0.0;
   r[0] -= an * (w[0] + lp->mu * nUT);
   r[1] -= an * w[1];
   r[2] -= an * w[2];
@@ -786,6 +845,8 @@ int oracle_local_state_init(const double* W, SolverOptions* o, double* state) {
       oracle_rho_split_spectral_norm_cond(W, state + 1);
     else if (s == SICONOS_FRICTION_3D_NSN_FORMULATION_RHO_STRATEGY_SPLIT_SPECTRAL_NORM)
       oracle_rho_split_spectral_norm(W, state + 1);
+    else if (s == SICONOS_FRICTION_3D_NSN_FORMULATION_RHO_STRATEGY_SPECTRAL_NORM)
+      oracle_rho_spectral_norm(W, state + 1);
     else if (s == SICONOS_FRICTION_3D_NSN_FORMULATION_RHO_STRATEGY_CONSTANT)
       state[1] = state[2] = state[3] = o->dparam[SICONOS_FRICTION_3D_NSN_RHO];
     else
@@ -853,8 +914,8 @@ static int tm_check(tol_manager* tm, SolverOptions* lo, double full, double incr
 }
 
 /* ------------------------------------------------------------------------ */
-/* fc3d_nsgs — fc3d_nsgs.c:560-917 (general loop; shuffle and freezing are
- * not supported by the GPU solver and are refused here too)                */
+/* fc3d_nsgs — fc3d_nsgs.c:560-917 (general loop, contact freezing of
+ * fc3d_nsgs.c:719-800 included; libc rand() shuffles are refused)          */
 /* ------------------------------------------------------------------------ */
 void oracle_fc3d_nsgs(FrictionContactProblem* problem, double* reaction, double* velocity,
                       int* info, SolverOptions* options, const int* order) {
@@ -865,10 +926,11 @@ void oracle_fc3d_nsgs(FrictionContactProblem* problem, double* reaction, double*
   double norm_r = 1e24;
   if (options->numberOfInternalSolvers < 1) { *info = NUMERICS_ERR_INVALID_ARGUMENT; return; }
   SolverOptions* lo = options->internalSolvers[0];
-  if (options->iparam[SICONOS_NSGS_SHUFFLE] != 0 || options->iparam[SICONOS_NSGS_FREEZING_CONTACT] > 0) {
+  if (options->iparam[SICONOS_NSGS_SHUFFLE] != 0) {
     *info = NUMERICS_ERR_NOT_IMPLEMENTED;
     return;
   }
+  const int nfreeze = options->iparam[SICONOS_NSGS_FREEZING_CONTACT];
   int ee = options->iparam[SICONOS_NSGS_ERROR_EVALUATION_TYPE];
   if (ee < 0 || ee > 3) { *info = NUMERICS_ERR_INVALID_ARGUMENT; return; }
   SparseBlockStructuredMatrix* M = problem->M->matrix1;
@@ -889,14 +951,34 @@ void oracle_fc3d_nsgs(FrictionContactProblem* problem, double* reaction, double*
     if (st) { *info = st; free(state); free(diag); return; }
   }
 
+  /* fc3d_nsgs.c:302-313 and :712: freeze counters and the last light-error
+   * term of each contact (the term a frozen contact adds again) */
+  unsigned* frz = nfreeze > 0 ? (unsigned*)calloc(nc, sizeof(unsigned)) : NULL;
+  double* le2 = (double*)calloc(nc, sizeof(double));
   int iter = 0, notconv = 1;
   double incr = INFINITY, full = INFINITY;
   double lr[3];
   while (iter < itermax && notconv > 0) {
     ++iter;
     double light_sum = 0.0;
+    /* fc3d_nsgs.c:721-735 */
+    const double ftmp = (double)nc * (double)nc / 1000.;
+    const double crit1 = tolerance * tolerance / ftmp;
+    const double crit2 = norm_r * norm_r / ftmp;
+    if (nfreeze > 0) {
+      unsigned nfrozen = 0;
+      for (size_t c = 0; c < nc; c++)
+        if (frz[c] > 0) nfrozen++;
+      if (nfrozen >= nc - 1)
+        for (size_t c = 0; c < nc; c++) frz[c] = 0;
+    }
     for (size_t i = 0; i < nc; ++i) {
       size_t c = order ? (size_t)order[i] : i;
+      if (nfreeze > 0 && frz[c] > 0) { /* fc3d_nsgs.c:749-756 */
+        frz[c] -= 1;
+        if (g_mut != ORACLE_MUT_FREEZE_NO_STALE) light_sum += le2[c];
+        continue;
+      }
       /* fc3d_nsgs_update: q_loc = q_c + sum_{j != c} W_cj r_j */
       double ql[3] = {problem->q[3 * c], problem->q[3 * c + 1], problem->q[3 * c + 2]};
       oracle_SBM_row_prod_no_diag_3x3(M, c, reaction, ql);
@@ -907,7 +989,14 @@ void oracle_fc3d_nsgs(FrictionContactProblem* problem, double* reaction, double*
         for (int k = 0; k < 3; k++) lr[k] = omega * lr[k] + (1.0 - omega) * reaction[3 * c + k];
       double d0 = reaction[3 * c] - lr[0], d1 = reaction[3 * c + 1] - lr[1],
              d2 = reaction[3 * c + 2] - lr[2];
-      light_sum += d0 * d0 + d1 * d1 + d2 * d2;
+      le2[c] = d0 * d0 + d1 * d1 + d2 * d2;
+      light_sum += le2[c];
+      if (nfreeze > 0) { /* fc3d_nsgs.c:776-800 */
+        double sq = lr[0] * lr[0] + lr[1] * lr[1] + lr[2] * lr[2];
+        int relative = le2[c] <= crit1 * sq;
+        int small = sq <= crit2;
+        if ((relative || small) && iter >= 10) frz[c] = (unsigned)nfreeze;
+      }
       if (options->iparam[SICONOS_NSGS_FILTER_LOCAL_SOLUTION] == 1) {
         double res = lo->dparam[SICONOS_DPARAM_RESIDU];
         if (!(isnan(res) || isinf(res) || res > 1.0)) memcpy(&reaction[3 * c], lr, sizeof lr);
@@ -969,6 +1058,8 @@ void oracle_fc3d_nsgs(FrictionContactProblem* problem, double* reaction, double*
   if (lo) lo->dparam[SICONOS_DPARAM_TOL] = tm0.local_orig;
   free(state);
   free(diag);
+  free(frz);
+  free(le2);
 }
 
 /* fc3d_driver.c:51-145 */

@@ -29,7 +29,11 @@ enum oracle_mutation {
   ORACLE_MUT_ERROR_DROP_MU = 3,     /* drops mu*|u_t| from the residual          */
   ORACLE_MUT_CONE_SIGN = 4,         /* boundary case of the cone projection off  */
   ORACLE_MUT_COLOUR_OFF_BY_ONE = 5, /* colouring ignores the first neighbour     */
-  ORACLE_MUT_GEPP_PIVOT = 6         /* 3x3 solve without row exchange            */
+  ORACLE_MUT_GEPP_PIVOT = 6,        /* 3x3 solve without row exchange            */
+  ORACLE_MUT_PROJ_DROP_MU = 7,      /* OC_PROJ step without the mu*|u_t| term    */
+  ORACLE_MUT_PROJ_RHO = 8,          /* OC_PROJ step with rho = 1/W11, not 1/W00  */
+  ORACLE_MUT_EIG_PHASE = 9,         /* eig_3x3 with the angle's sign flipped     */
+  ORACLE_MUT_FREEZE_NO_STALE = 10   /* a frozen contact adds no light-error term */
 };
 ORACLE_API void oracle_set_mutation(int m);
 /* Work counters: [0] Newton calls, [1] Newton iterations, [2] PLI calls,
@@ -66,6 +70,8 @@ ORACLE_API void oracle_alart_curnier_dual(const double* R, const double* u, doub
 /* fc3d_AlartCurnier_functions.c:755-812 */
 ORACLE_API void oracle_rho_split_spectral_norm(const double* W, double* rho);
 ORACLE_API void oracle_rho_split_spectral_norm_cond(const double* W, double* rho);
+ORACLE_API void oracle_eig_3x3(const double* a, double* eig);
+ORACLE_API void oracle_rho_spectral_norm(const double* W, double* rho);
 
 /* fc3d_compute_error.c:42-56 */
 ORACLE_API void oracle_unitary_compute_and_add_error(const double* r, const double* u, double mu,

@@ -43,6 +43,8 @@ def load():
         "oracle_alart_curnier_dual": (None, [dp, dp, C.c_double, dp, dp, dp, dp]),
         "oracle_rho_split_spectral_norm": (None, [dp, dp]),
         "oracle_rho_split_spectral_norm_cond": (None, [dp, dp]),
+        "oracle_eig_3x3": (None, [dp, dp]),
+        "oracle_rho_spectral_norm": (None, [dp, dp]),
         "oracle_unitary_compute_and_add_error": (None, [dp, dp, C.c_double, dp]),
         "oracle_fc3d_compute_error": (C.c_int, [C.c_void_p, dp, dp, C.c_double, C.c_double, dp]),
         "oracle_nsgs_options_new": (C.c_void_p, []),
@@ -89,14 +91,21 @@ def i64(a):
     return a.ctypes.data_as(lp)
 
 
+CALLBACK_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_size_t, dp, dp, C.c_double, C.c_void_p)
+
+
+class _Callback(C.Structure):
+    _fields_ = [("env", C.c_void_p), ("collectStatsIteration", CALLBACK_FN)]
+
+
 class _SO(C.Structure):
-    # leading fields of SolverOptions (fc3d_numerics.h), enough to reach the
-    # parameter arrays and the internal solver
+    # SolverOptions (fc3d_numerics.h)
     _fields_ = [("solverId", C.c_int), ("isSet", C.c_bool), ("iSize", C.c_size_t),
                 ("iparam", ip), ("dSize", C.c_size_t), ("dparam", dp), ("filterOn", C.c_bool),
                 ("dWorkSize", C.c_size_t), ("dWork", dp), ("iWorkSize", C.c_size_t),
                 ("iWork", ip), ("numberOfInternalSolvers", C.c_size_t),
-                ("internalSolvers", C.POINTER(C.c_void_p))]
+                ("internalSolvers", C.POINTER(C.c_void_p)), ("callback", C.POINTER(_Callback)),
+                ("solverParameters", C.c_void_p), ("solverData", C.c_void_p)]
 
 
 class Options:
@@ -130,8 +139,27 @@ class Options:
     def set_local_solver(self, sid):
         self.lib.oracle_set_local_solver(self.p, sid)
 
+    def set_callback(self, fn):
+        """fn(reaction, velocity, error) after every sweep (copies of the
+        arrays), as the reference's collectStatsIteration is called."""
+        if fn is None:
+            self._s.callback = None
+            self._cb = None
+            return
+
+        def tramp(env, size, r, u, err, extra):
+            fn(np.ctypeslib.as_array(r, shape=(size,)).copy(),
+               np.ctypeslib.as_array(u, shape=(size,)).copy(), err)
+
+        f = CALLBACK_FN(tramp)
+        cb = _Callback(None, f)
+        self._cb = (f, cb)
+        self._s.callback = C.pointer(cb)
+
     def __del__(self):
         if getattr(self, "p", None) and self.owner is None:
+            if getattr(self, "_cb", None):
+                self._s.callback = None
             self.lib.oracle_options_free(self.p)
             self.p = None
 

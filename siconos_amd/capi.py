@@ -59,6 +59,10 @@ CALLBACK_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_size_t, c_double_p, c_double_p, 
                           C.c_void_p)
 
 
+# fc3d_hip_sweep_observer: (env, sweeps_done, incr_error)
+OBSERVER_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_double)
+
+
 class Callback(C.Structure):
     _fields_ = [("env", C.c_void_p), ("collectStatsIteration", CALLBACK_FN)]
 
@@ -89,7 +93,8 @@ class HipParams(C.Structure):
                 ("schedule", C.c_int), ("sweeps_per_launch", C.c_int), ("fp32_tol", C.c_double),
                 ("fp32_local_tol", C.c_double), ("instrument", C.c_int),
                 ("accel_depth", C.c_int), ("accel_every", C.c_int), ("accel_reg", C.c_double),
-                ("velocity_form", C.c_int), ("fp32_local_solver", C.c_int)]
+                ("velocity_form", C.c_int), ("fp32_local_solver", C.c_int),
+                ("freeze_contacts", C.c_int)]
 
 
 class HipStats(C.Structure):
@@ -231,7 +236,11 @@ SIGNATURES = {
     "fc3d_hip_get_reaction": (C.c_int, [H_p, c_double_p]),
     "fc3d_hip_get_velocity": (C.c_int, [H_p, c_double_p]),
     "fc3d_hip_solve": (C.c_int, [H_p, C.POINTER(HipParams), C.POINTER(HipStats)]),
+    "fc3d_hip_solve_observed": (C.c_int, [H_p, C.POINTER(HipParams), C.POINTER(HipStats),
+                                          OBSERVER_FN, C.c_void_p]),
     "fc3d_hip_compute_error": (C.c_int, [H_p, C.c_double, c_double_p]),
+    "fc3d_hip_resident_clear": (None, [NM_p]),
+    "fc3d_hip_resident_stats": (None, [c_int64_p, c_int64_p, c_int64_p]),
     "fc3d_hip_gemv": (C.c_int, [H_p, c_double_p, c_double_p]),
     "fc3d_hip_alart_curnier": (C.c_int, [C.c_int, c_double_p, c_double_p, c_double_p, c_double_p,
                                          C.c_int, c_double_p, c_double_p, c_double_p]),

@@ -518,6 +518,10 @@ int fc3d_hip_batch_solve(fc3d_hip_batch* b, const fc3d_hip_params* p, fc3d_hip_b
     set_error("fc3d_hip_batch_solve: Anderson acceleration is not available for batches");
     return NUMERICS_ERR_NOT_IMPLEMENTED;
   }
+  if (p->freeze_contacts != 0) {
+    set_error("fc3d_hip_batch_solve: contact freezing is not available in the batched solver");
+    return NUMERICS_ERR_NOT_IMPLEMENTED;
+  }
   DevParams d;
   std::memset(&d, 0, sizeof d);
   d.lp.solver = p->local_solver;

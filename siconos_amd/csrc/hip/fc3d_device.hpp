@@ -221,12 +221,60 @@ __device__ __forceinline__ void ac_function(int formulation, const T* R, const T
     ac_std<T, AB>(R, u, mu, rho, F, A, B);
 }
 
-// fc3d_AlartCurnier_functions.c:784-812 (quirk sw - W21 + W12 kept) and
-// 755-782 (_cond variant); strategy ids of FrictionContact_options.h:329-340.
+// op3x3.h:1065-1111 — eigenvalues of a symmetric column-major 3x3 matrix by
+// the trigonometric method, in the reference's operation order.  A diagonal
+// matrix returns its diagonal as stored (not sorted); otherwise eig[0] is the
+// largest.  acos and cos are the device library's, so eig may differ from a
+// host libm's in the last bit.
+template <typename T>
+__device__ __forceinline__ void eig_3x3(const T* a, T* eig) {
+  const T a00 = a[0], a10 = a[1], a20 = a[2], a01 = a[3], a11 = a[4], a21 = a[5];
+  const T a02 = a[6], a12 = a[7], a22 = a[8];
+  const T pi = T(3.14159265358979323846);
+  const T p1 = a01 * a01 + a02 * a02 + a12 * a12;
+  if (p1 == T(0)) {
+    eig[0] = a00;
+    eig[1] = a11;
+    eig[2] = a22;
+    return;
+  }
+  const T q = (a00 + a11 + a22) / T(3);
+  const T p2 = (a00 - q) * (a00 - q) + (a11 - q) * (a11 - q) + (a22 - q) * (a22 - q) + T(2) * p1;
+  const T p = sqrt(p2 / T(6));
+  const T ip = T(1) / p;
+  const T b00 = ip * (a00 - q), b11 = ip * (a11 - q), b22 = ip * (a22 - q);
+  const T b01 = ip * a01, b02 = ip * a02, b10 = ip * a10, b12 = ip * a12, b20 = ip * a20;
+  const T b21 = ip * a21;
+  const T det = b00 * b11 * b22 + b01 * b12 * b20 + b02 * b10 * b21 - b00 * b12 * b21 -
+                b01 * b10 * b22 - b02 * b11 * b20;
+  const T r = det / T(2);
+  T phi;
+  if (r <= T(-1))
+    phi = pi / T(3);
+  else if (r >= T(1))
+    phi = T(0);
+  else
+    phi = acos(r) / T(3);
+  eig[0] = q + T(2) * p * cos(phi);
+  eig[2] = q + T(2) * p * cos(phi + (T(2) * pi / T(3)));
+  eig[1] = T(3) * q - eig[0] - eig[2];
+}
+
+// fc3d_AlartCurnier_functions.c:784-812 (quirk sw - W21 + W12 kept),
+// 755-782 (_cond variant) and 814-829 (spectral norm: 1/eig[0] on all three);
+// strategy ids of FrictionContact_options.h:329-340.
 template <typename T>
 __device__ __forceinline__ void compute_rho(int strategy, T rho_constant, const T* W, T* rho) {
   if (strategy == 0) {
     rho[0] = rho[1] = rho[2] = rho_constant;
+    return;
+  }
+  if (strategy == 1) {
+    T eig[3];
+    eig_3x3<T>(W, eig);
+    rho[0] = T(1) / eig[0];
+    rho[1] = rho[0];
+    rho[2] = rho[0];
     return;
   }
   const T sw = W[1 + 1 * 3] + W[2 + 2 * 3];

@@ -36,6 +36,15 @@ namespace fc3d {
 int colour_on_device(int nc, const int* d_rp, const int* d_ci, int* d_colour, hipStream_t s,
                      int* ncolours_out, std::vector<int>* host_colour);
 
+// Contact freezing: the number of the slice's contacts frozen after this
+// sweep, for the reset rule of the next one (fc3d_nsgs.c:726-735).
+template <typename T>
+__device__ __forceinline__ void count_frozen(const SweepPtrs<T>& P, int s, int lane, int len) {
+  const bool f = len >= 0 && P.frz[s * 64 + lane] > 0;
+  const unsigned long long b = __builtin_amdgcn_ballot_w64(f);
+  if (lane == 0) P.fcnt[s] = __builtin_popcountll(b);
+}
+
 // One colour of one sweep: fc3d_nsgs.c:737-822 for the contacts of slices
 // [s_begin, s_end), which share no body.
 template <typename T, int U, int PF, int MINW>
@@ -64,6 +73,7 @@ __global__ __launch_bounds__(256, MINW) void k_sweep(SweepPtrs<T> P,
   if (stop || phase != want_phase) return;
   double light, nr2;
   sweep_slot_loaded<T, U, PF>(P, sc, s, lane, tol, len, in, koff, kmax, &light, &nr2);
+  if (P.fcnt) count_frozen(P, s, lane, len);
   light = wave_sum(light);
   nr2 = wave_sum(nr2);
   if (lane == 0) {
@@ -178,6 +188,7 @@ __global__ __launch_bounds__(256, MINW) void k_sweep_df(SweepPtrs<T> P,
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     double light, nr2;
     sweep_slot<T, U, PF>(P, prm, s, lane, tol, &light, &nr2);
+    if (P.fcnt) count_frozen(P, s, lane, P.len[s * 64 + lane]);
     light = wave_sum(light);
     nr2 = wave_sum(nr2);
     if (lane == 0) {
@@ -253,14 +264,29 @@ __global__ __launch_bounds__(256) void k_residual(SweepPtrs<double> P, const Sol
 // End of a sweep: light error (fc3d_nsgs.c:90-96) and the convergence test of
 // the chosen evaluation mode (fc3d_nsgs.c:826-863).  In the velocity form the
 // tail launch does the same work instead, when the sweep has a tail.
+// With contact freezing (fcnt not null), the frozen contacts are counted
+// too: when at least nc - 1 are frozen, every counter reads 0 in the next
+// sweep (fc3d_nsgs.c:726-735).
 __global__ __launch_bounds__(1024) void k_finalize_sweep(const DevParams* prm, SolveState* st,
                                                          const double* part, int nslices,
-                                                         int want_phase) {
+                                                         int want_phase, const int* fcnt) {
   __shared__ double sh[2 * 16];
+  __shared__ int nfrozen;
   if (st->stop || st->phase != want_phase) return;
   double sums[2];
   block_sums<2>(part, nslices, sh, sums);
-  if (threadIdx.x == 0) end_sweep_state(prm, st, sums[0], sums[1], want_phase);
+  if (fcnt) {
+    if (threadIdx.x == 0) nfrozen = 0;
+    __syncthreads();
+    int c = 0;
+    for (int k = threadIdx.x; k < nslices; k += blockDim.x) c += fcnt[k];
+    atomicAdd(&nfrozen, c);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    end_sweep_state(prm, st, sums[0], sums[1], want_phase);
+    if (fcnt) st->unfreeze = nfrozen >= prm->nc - 1 ? 1 : 0;
+  }
 }
 
 // fc3d_compute_error.c:100-115 and, unless `plain`, the tolerance manager
@@ -526,6 +552,13 @@ struct fc3d_hip_problem {
   float *d_H32 = nullptr, *d_minv32 = nullptr;
   float4* d_V32 = nullptr;
   std::vector<double> vel_bytes64, vel_bytes32;  // per colour
+  // contact freezing (allocated on first use): per-slot counters and last
+  // light-error terms, per-slice frozen counts; freeze = 0 when off
+  int freeze = 0, graph_freeze[2] = {0, 0};
+  double freeze_tmp = 1.0;
+  int* d_frz = nullptr;
+  double* d_le2 = nullptr;
+  int* d_fcnt = nullptr;
 };
 
 namespace fc3d {
@@ -548,6 +581,10 @@ static SweepPtrs<double> ptrs64(fc3d_hip_problem* h, double* part) {
   P.len = h->len; P.dpos = h->dpos; P.koff = h->koff; P.kmax = h->kmax; P.col = h->col;
   P.vals = h->vals; P.diag = h->diag; P.q = h->q; P.mu = h->mu; P.r = h->r; P.u = h->u;
   P.pli = h->pli; P.part = part;
+  if (h->freeze > 0) {
+    P.frz = h->d_frz; P.le2 = h->d_le2; P.fcnt = h->d_fcnt; P.fst = h->st;
+    P.freeze = h->freeze; P.freeze_tmp = h->freeze_tmp;
+  }
   return P;
 }
 static SweepPtrs<float> ptrs32(fc3d_hip_problem* h) {
@@ -842,7 +879,8 @@ static void enqueue_sweep(fc3d_hip_problem* h, int phase, int schedule, hipEvent
   }
   if (!h->vel_on || vel_tail_begin(h, phase) >= h->ncolours)
     hipLaunchKernelGGL(k_finalize_sweep, dim3(1), dim3(1024), 0, h->stream, h->prm, h->st,
-                       h->part, h->nslices, phase);
+                       h->part, h->nslices, phase,
+                       (const int*)(h->freeze > 0 && !phase ? h->d_fcnt : nullptr));
   if (!phase) {
     hipLaunchKernelGGL(k_residual, dim3((h->nslices + 3) / 4), dim3(256), 0, h->stream,
                        ptrs64(h, h->part2), h->st, h->nslices, 0, nullptr, 0);
@@ -1038,7 +1076,8 @@ static int reset_accel(fc3d_hip_problem* h) {
 static int build_graph(fc3d_hip_problem* h, int phase, int sweeps, int schedule) {
   if (h->graph[phase] && h->graph_sweeps[phase] == sweeps && h->graph_schedule[phase] == schedule &&
       h->graph_accel[phase] == h->accel_on && h->graph_vel[phase] == h->vel_on &&
-      h->graph_tail[phase] == vel_tail_begin(h, phase) && h->graph_pli32_off[phase] == h->pli32_off)
+      h->graph_tail[phase] == vel_tail_begin(h, phase) && h->graph_pli32_off[phase] == h->pli32_off &&
+      h->graph_freeze[phase] == h->freeze)
     return 0;
   if (h->graph[phase]) {
     hipGraphExecDestroy(h->graph[phase]);
@@ -1056,6 +1095,7 @@ static int build_graph(fc3d_hip_problem* h, int phase, int sweeps, int schedule)
   h->graph_vel[phase] = h->vel_on;
   h->graph_tail[phase] = vel_tail_begin(h, phase);
   h->graph_pli32_off[phase] = h->pli32_off;
+  h->graph_freeze[phase] = h->freeze;
   return 0;
 }
 
@@ -1449,7 +1489,7 @@ void fc3d_hip_problem_destroy(fc3d_hip_problem* h) {
   void* dfp[] = {h->dep_ptr, h->dep_idx, h->qlist, h->done,    h->df,    h->d_block_dst,
                  h->d_o2s,    h->d_cps,    h->d_ba,   h->d_bb,   h->d_inc_ptr, h->d_inc, h->d_H,
                  h->d_minv,   h->d_vfree,  h->d_V,    h->d_H32,     h->d_minv32, h->d_V32,
-                 h->d_F,      h->d_F32};
+                 h->d_F,      h->d_F32,    h->d_frz,  h->d_le2,  h->d_fcnt};
   for (void* p : dfp)
     if (p) hipFree(p);
   if (h->stream) hipStreamDestroy(h->stream);
@@ -1816,6 +1856,11 @@ int fc3d_hip_problem_set_model(fc3d_hip_problem* h, fc3d_hip_mlp* m) {
 }
 
 int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats* s) {
+  return fc3d_hip_solve_observed(h, p, s, nullptr, nullptr);
+}
+
+int fc3d_hip_solve_observed(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats* s,
+                            fc3d_hip_sweep_observer observer, void* env) {
   if (!h || !p || !s) return NUMERICS_ERR_NULL_POINTER;
   const int ls = p->local_solver;
   if (!(ls == 565 || ls == 551 || ls == 550 || ls == 553 || ls == 552)) {
@@ -1831,7 +1876,7 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
     set_error("fc3d_hip_solve: unknown error evaluation %d", p->error_evaluation);
     return NUMERICS_ERR_INVALID_ARGUMENT;
   }
-  if (p->rho_strategy != 0 && p->rho_strategy != 2 && p->rho_strategy != 3 &&
+  if ((p->rho_strategy < 0 || p->rho_strategy > 3) &&
       (ls == 565 || ls == 551 || ls == 550 || ls32 == 565 || ls32 == 551 || ls32 == 550)) {
     set_error("fc3d_hip_solve: rho strategy %d not supported", p->rho_strategy);
     return NUMERICS_ERR_NOT_IMPLEMENTED;
@@ -1855,6 +1900,13 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
     set_error("fc3d_hip_solve: velocity_form with a learned start is not available");
     return NUMERICS_ERR_NOT_IMPLEMENTED;
   }
+  if (p->freeze_contacts < 0 ||
+      (p->freeze_contacts > 0 && (p->precision != 0 || p->velocity_form || p->accel_depth > 0 ||
+                                  p->warm_start >= 2))) {
+    set_error("fc3d_hip_solve: contact freezing (%d) needs FP64 sweeps on the rows of W, without "
+              "acceleration or a learned start", p->freeze_contacts);
+    return NUMERICS_ERR_NOT_IMPLEMENTED;
+  }
   if (p->warm_start >= 2 && !h->model) {
     set_error("fc3d_hip_solve: warm_start %d needs a model (fc3d_hip_problem_set_model)",
               p->warm_start);
@@ -1866,7 +1918,12 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
   s->num_colours = h->ncolours;
   s->warm_start_used = p->warm_start == 1 ? 1 : 0;
 
-  const DevParams dp = make_params(p);
+  h->freeze = 0;  // the autotuner's trial sweeps do not freeze
+  h->freeze_tmp = (double)h->nc * (double)h->nc / 1000.;
+  DevParams dp = make_params(p);
+  dp.freeze = p->freeze_contacts;
+  dp.nc = h->nc;
+  dp.freeze_tmp = h->freeze_tmp;
   FC3D_HIP_TRY(hipMemcpyAsync(h->prm, &dp, sizeof dp, hipMemcpyHostToDevice, h->stream));
   if (p->warm_start == 1)
     FC3D_HIP_TRY(hipMemsetAsync(h->r, 0, sizeof(double4) * (size_t)h->nslots, h->stream));
@@ -1919,8 +1976,21 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
     h->vel_on = 1;
   }
 
+  if (p->freeze_contacts > 0) {
+    // fc3d_nsgs.c:302-313, 712: counters and light-error terms start at 0
+    if (!h->d_frz && ((rc = dev_alloc(&h->d_frz, (size_t)h->nslots)) ||
+                      (rc = dev_alloc(&h->d_le2, (size_t)h->nslots)) ||
+                      (rc = dev_alloc(&h->d_fcnt, (size_t)h->nslices))))
+      return rc;
+    FC3D_HIP_TRY(hipMemsetAsync(h->d_frz, 0, sizeof(int) * (size_t)h->nslots, h->stream));
+    FC3D_HIP_TRY(hipMemsetAsync(h->d_le2, 0, sizeof(double) * (size_t)h->nslots, h->stream));
+    FC3D_HIP_TRY(hipMemsetAsync(h->d_fcnt, 0, sizeof(int) * (size_t)h->nslices, h->stream));
+    h->freeze = p->freeze_contacts;
+  }
+
   SolveState s0;
   std::memset(&s0, 0, sizeof s0);
+  s0.norm_r = 1e24;  // fc3d_nsgs.c:575
   s0.stop = p->max_iter <= 0 ? 1 : 0;
   s0.notconv = 1;
   s0.freq = p->error_frequency;
@@ -2013,6 +2083,15 @@ int fc3d_hip_solve(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_hip_stats
         FC3D_HIP_TRY(hipGraphLaunch(h->graph[phase], h->stream));
         FC3D_HIP_TRY(hipMemcpyAsync(&cur, h->st, sizeof cur, hipMemcpyDeviceToHost, h->stream));
         FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
+      }
+      if (observer) {
+        // the iterate is readable in FP64 while the observer runs
+        if (phase == 1) {
+          hipLaunchKernelGGL(k_r32_to_r64, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0,
+                             h->stream, h->r32, h->r, h->pli32, h->pli, ns);
+          FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
+        }
+        observer(env, cur.iter, cur.incr);
       }
       if (cur.stop) break;
     }

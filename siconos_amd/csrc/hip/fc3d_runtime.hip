@@ -69,6 +69,7 @@ void fc3d_hip_params_default(fc3d_hip_params* p) {
   p->accel_reg = 1e-10;
   p->velocity_form = 0;
   p->fp32_local_solver = 0;
+  p->freeze_contacts = 0;
 }
 
 }  // extern "C"

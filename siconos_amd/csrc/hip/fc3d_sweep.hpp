@@ -28,6 +28,9 @@ struct DevParams {
   int filter;
   int relaxation;
   int rho_strategy;
+  int freeze;           // contact freezing: sweeps a frozen contact is skipped (0: off)
+  int nc;               // contacts of the problem (freezing's reset rule)
+  double freeze_tmp;    // nc * nc / 1000 (fc3d_nsgs.c:722)
 };
 
 struct SolveState {
@@ -45,6 +48,9 @@ struct SolveState {
   double local_tol;
   double norm_q;
   double nf;      // sqrt of the last sweep's sum |r_new - r_old|^2
+  double norm_r;  // |r| after the last light-error evaluation (1e24 before)
+  int unfreeze;   // contact freezing: every counter reads 0 in this sweep
+  int pad2;
 };
 
 template <typename T>
@@ -64,6 +70,15 @@ struct SweepPtrs {
                     // local solver keeps none (550, 551, 552): not loaded, not stored
   double* part;     // per-slice partial sums
   typename vec4<T>::type* f = nullptr;  // if set: r_new - r_old of the sweep
+  // contact freezing (FP64 sweeps; null when off): per-slot counters of the
+  // sweeps still to skip, the slot's last light-error term, per-slice counts
+  // of frozen slots after the sweep, and the solve state the criteria read
+  int* frz = nullptr;
+  double* le2 = nullptr;
+  int* fcnt = nullptr;
+  const SolveState* fst = nullptr;
+  int freeze = 0;
+  double freeze_tmp = 1.0;
 };
 
 // Adds the value of the lane that DPP control CTRL selects (all lanes active).
@@ -184,6 +199,9 @@ __device__ inline void end_sweep_state(const DevParams* prm, SolveState* st, dou
   const double nr = sqrt(nr2);
   if (fabs(nr) > DBL_EPSILON) incr /= nr;
   x.incr = incr;
+  if (want_phase == 0 && (mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT ||
+                          mode == SICONOS_NSGS_ERROR_EVALUATION_LIGHT_WITH_FULL_FINAL))
+    x.norm_r = nr;  // calculateLightError keeps |r| (fc3d_nsgs.c:90-96)
   if (want_phase == 1) {
     // FP32 phase: hand over to FP64 sweeps once the FP32 light error is small
     x.fp32_iter += 1;
@@ -374,6 +392,18 @@ __device__ __forceinline__ void finish_slot(const SweepPtrs<T>& P, const StepCon
   const int slot = s * 64 + lane;
   LocalProblem<T>& lp = in.lp;
   const typename vec4<T>::type ro = in.ro;
+  if (P.frz) {
+    // fc3d_nsgs.c:749-756: a frozen contact keeps its reaction and adds its
+    // last light-error term again
+    const int f0 = P.fst->unfreeze ? 0 : P.frz[slot];
+    if (f0 > 0) {
+      P.frz[slot] = f0 - 1;
+      *light = P.le2[slot];
+      *nr2 = double(ro.x * ro.x + ro.y * ro.y + ro.z * ro.z);
+      if (out_r) *out_r = ro;
+      return;
+    }
+  }
   T pli = in.pli;
   lp.q[0] = in.ql[0]; lp.q[1] = in.ql[1]; lp.q[2] = in.ql[2];
   const T rold[3] = {ro.x, ro.y, ro.z};
@@ -389,6 +419,19 @@ __device__ __forceinline__ void finish_slot(const SweepPtrs<T>& P, const StepCon
   }
   const T d0 = rold[0] - lr[0], d1 = rold[1] - lr[1], d2 = rold[2] - lr[2];
   *light = double(d0 * d0 + d1 * d1 + d2 * d2);
+  if (P.frz) {
+    // fc3d_nsgs.c:723-724, 776-800: freeze the contact for P.freeze sweeps
+    // when its step is small against its reaction, or its reaction small
+    // against |r|, from the 10th sweep on
+    const SolveState* fs = P.fst;
+    const double wt = fs->working_tol, nr = fs->norm_r;
+    const double crit1 = wt * wt / P.freeze_tmp, crit2 = nr * nr / P.freeze_tmp;
+    const double l2 = *light;
+    const double sq = double(lr[0] * lr[0] + lr[1] * lr[1] + lr[2] * lr[2]);
+    P.le2[slot] = l2;
+    const bool relative = l2 <= crit1 * sq, small = sq <= crit2;
+    P.frz[slot] = ((relative || small) && fs->iter + 1 >= 10) ? P.freeze : 0;
+  }
   const bool accept = !(c.filter && (isnan(res) || isinf(res) || res > T(1)));
   typename vec4<T>::type out = ro;
   if (accept) { out.x = lr[0]; out.y = lr[1]; out.z = lr[2]; }

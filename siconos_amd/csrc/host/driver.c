@@ -8,8 +8,9 @@
 #include "fc3d_hip.h"
 #include "fc3d_numerics.h"
 
-int fc3d_sbm_to_csr(const SparseBlockStructuredMatrix* A, int64_t** rp, int64_t** ci,
-                    double** vals);
+int fc3d_resident_acquire(FrictionContactProblem* problem, int device, int mode,
+                          fc3d_hip_problem** out, int* token);
+void fc3d_resident_release(fc3d_hip_problem* h, int token);
 
 int fc3d_checkTrivialCase(FrictionContactProblem* problem, double* velocity, double* reaction,
                           SolverOptions* options) {
@@ -81,43 +82,25 @@ SolverOptions* fc3d_solver_options_create(solver_id_t id) {
   return solver_options_create(id);
 }
 
-/* Builds a resident GPU copy of problem->M (SBM, or dense converted to SBM). */
-int fc3d_problem_to_device(FrictionContactProblem* problem, int device, fc3d_hip_problem** h) {
-  SparseBlockStructuredMatrix* S = NULL;
-  int own = 0;
-  if (problem->M->storageType == NM_SPARSE_BLOCK) {
-    S = problem->M->matrix1;
-  } else if (problem->M->storageType == NM_DENSE) {
-    S = SBM_from_dense_3x3(problem->numberOfContacts, problem->M->matrix0);
-    own = 1;
-  } else {
-    return NUMERICS_ERR_NOT_IMPLEMENTED;
-  }
-  if (!S) return NUMERICS_ERR_INVALID_ARGUMENT;
-  int64_t *rp, *ci;
-  double* vals;
-  int rc = fc3d_sbm_to_csr(S, &rp, &ci, &vals);
-  if (!rc)
-    rc = fc3d_hip_problem_create(problem->numberOfContacts, rp, ci, vals, problem->q, problem->mu,
-                                 device, h);
-  free(rp);
-  free(ci);
-  free(vals);
-  if (own) SBM_free_all(S);
-  return rc;
-}
-
 int fc3d_compute_error(FrictionContactProblem* problem, double* z, double* w, double tolerance,
                        SolverOptions* options, double norm, double* error) {
-  (void)options;
   if (!problem || !z || !w || !error || !problem->M || !problem->q || !problem->mu)
     return NUMERICS_ERR_NULL_POINTER;
+  /* the device problem a solve on the same matrix left resident is reused;
+   * the residency option is read from FC3D_NSGS_HIP options only */
+  int mode = 0, device = 0;
+  if (options && options->solverId == SICONOS_FRICTION_3D_NSGS_HIP && options->dparam &&
+      options->iparam) {
+    mode = (int)options->dparam[SICONOS_FRICTION_3D_NSGS_HIP_RESIDENCY];
+    device = options->iparam[SICONOS_FRICTION_3D_NSGS_HIP_DEVICE];
+  }
   fc3d_hip_problem* h = NULL;
-  int rc = fc3d_problem_to_device(problem, 0, &h);
+  int token = -1;
+  int rc = fc3d_resident_acquire(problem, device, mode, &h, &token);
   if (!rc) rc = fc3d_hip_set_reaction(h, z);
   if (!rc) rc = fc3d_hip_compute_error(h, norm, error);
   if (!rc) rc = fc3d_hip_get_velocity(h, w);
-  fc3d_hip_problem_destroy(h);
+  if (h) fc3d_resident_release(h, token);
   if (rc) return rc;
   return *error > tolerance ? 1 : 0;
 }

@@ -122,6 +122,7 @@ NumericsMatrix* NM_create_from_data(int storageType, int size0, int size1, void*
 
 NumericsMatrix* NM_free(NumericsMatrix* m) {
   if (!m) return NULL;
+  fc3d_hip_resident_clear(m);  /* its address may be reused by another matrix */
   free(m->matrix0);
   SBM_free_all(m->matrix1);
   NSM_free(m->matrix2);

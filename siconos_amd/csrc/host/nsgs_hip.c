@@ -14,7 +14,33 @@
 #include "fc3d_hip.h"
 #include "fc3d_numerics.h"
 
-int fc3d_problem_to_device(FrictionContactProblem* problem, int device, fc3d_hip_problem** h);
+int fc3d_resident_acquire(FrictionContactProblem* problem, int device, int mode,
+                          fc3d_hip_problem** out, int* token);
+void fc3d_resident_release(fc3d_hip_problem* h, int token);
+
+/* The reference's per-sweep statistics callback (fc3d_nsgs.c:103-110) behind
+ * the device solve's observer: the current reaction and the velocity of the
+ * last full residual are copied to the caller's arrays first. */
+typedef struct {
+  fc3d_hip_problem* h;
+  SolverOptions* options;
+  FrictionContactProblem* problem;
+  double* reaction;
+  double* velocity;
+  int failed;
+} observer_env;
+
+static void observe_sweep(void* env, int sweeps_done, double incr_error) {
+  observer_env* e = (observer_env*)env;
+  (void)sweeps_done;
+  if (fc3d_hip_get_reaction(e->h, e->reaction) || fc3d_hip_get_velocity(e->h, e->velocity)) {
+    e->failed = 1;
+    return;
+  }
+  Callback* cb = e->options->callback;
+  cb->collectStatsIteration(cb->env, (size_t)(3 * e->problem->numberOfContacts), e->reaction,
+                            e->velocity, incr_error, NULL);
+}
 
 static double now_ms(void) {
   struct timespec ts;
@@ -27,8 +53,7 @@ int fc3d_nsgs_hip_params_from_options(const SolverOptions* o, fc3d_hip_params* p
   fc3d_hip_params_default(p);
   if (o->numberOfInternalSolvers < 1 || !o->internalSolvers || !o->internalSolvers[0])
     return NUMERICS_ERR_INVALID_OPTION;
-  if (o->iparam[SICONOS_NSGS_SHUFFLE] != 0 || o->iparam[SICONOS_NSGS_FREEZING_CONTACT] > 0)
-    return NUMERICS_ERR_NOT_IMPLEMENTED;
+  if (o->iparam[SICONOS_NSGS_SHUFFLE] != 0) return NUMERICS_ERR_NOT_IMPLEMENTED;
   if (o->iparam[SICONOS_FRICTION_3D_IPARAM_INTERNAL_ERROR_STRATEGY] !=
       SICONOS_NSGS_INTERNAL_ERROR_STRATEGY_GIVEN_VALUE)
     return NUMERICS_ERR_NOT_IMPLEMENTED;
@@ -65,6 +90,8 @@ int fc3d_nsgs_hip_params_from_options(const SolverOptions* o, fc3d_hip_params* p
   p->fp32_local_solver = (int)o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_FP32_LOCAL_SOLVER];
   if (o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_FP32_LOCAL_TOL] > 0.0)
     p->fp32_local_tol = o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_FP32_LOCAL_TOL];
+  p->freeze_contacts =
+      o->iparam[SICONOS_NSGS_FREEZING_CONTACT] > 0 ? o->iparam[SICONOS_NSGS_FREEZING_CONTACT] : 0;
   return 0;
 }
 
@@ -79,19 +106,32 @@ void fc3d_nsgs_hip(FrictionContactProblem* problem, double* reaction, double* ve
   }
   double t0 = now_ms();
   fc3d_hip_problem* h = NULL;
-  rc = fc3d_problem_to_device(problem, options->iparam[SICONOS_FRICTION_3D_NSGS_HIP_DEVICE], &h);
+  int token = -1;
+  rc = fc3d_resident_acquire(problem, options->iparam[SICONOS_FRICTION_3D_NSGS_HIP_DEVICE],
+                             (int)options->dparam[SICONOS_FRICTION_3D_NSGS_HIP_RESIDENCY], &h,
+                             &token);
   if (!rc) rc = fc3d_hip_set_reaction(h, reaction);
   /* warm starts 2 and 3 use the process-wide model (fc3d_hip_set_default_model) */
   if (!rc && p.warm_start >= 2) rc = fc3d_hip_problem_set_model(h, fc3d_hip_default_model());
   double t1 = now_ms();
   fc3d_hip_stats s;
-  if (!rc) rc = fc3d_hip_solve(h, &p, &s);
+  if (!rc) {
+    if (options->callback && options->callback->collectStatsIteration) {
+      /* one sweep per replay: the callback sees every sweep, as in the reference */
+      observer_env env = {h, options, problem, reaction, velocity, 0};
+      p.sweeps_per_launch = 1;
+      rc = fc3d_hip_solve_observed(h, &p, &s, observe_sweep, &env);
+      if (!rc && env.failed) rc = NUMERICS_ERR_DEVICE;
+    } else {
+      rc = fc3d_hip_solve(h, &p, &s);
+    }
+  }
   double t2 = now_ms();
   if (!rc) rc = fc3d_hip_get_reaction(h, reaction);
   if (!rc && p.error_evaluation != SICONOS_NSGS_ERROR_EVALUATION_LIGHT)
     rc = fc3d_hip_get_velocity(h, velocity);
   double t3 = now_ms();
-  fc3d_hip_problem_destroy(h);
+  if (h) fc3d_resident_release(h, token);
   if (rc) {
     fprintf(stderr, "fc3d_nsgs_hip: %s (%s)\n", numerics_error_string(rc), fc3d_hip_last_error());
     *info = rc;
@@ -142,6 +182,7 @@ void fc3d_nsgs_hip_set_default(SolverOptions* o) {
   o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_ACCEL_REG] = 1e-10;
   o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_FP32_LOCAL_SOLVER] = 0.0;
   o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_FP32_LOCAL_TOL] = 0.0;
+  o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_RESIDENCY] = 0.0;
 }
 
 static int nsgs_hip_solve_wrap(void* problem, double* reaction, double* velocity,

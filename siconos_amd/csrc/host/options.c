@@ -83,8 +83,11 @@ SolverOptions* solver_options_copy(SolverOptions* src) {
     solver_options_delete(o->internalSolvers[i]);
     o->internalSolvers[i] = solver_options_copy(src->internalSolvers[i]);
   }
-  /* pointer links, as the reference does */
-  o->callback = NULL;
+  /* pointer links shared with the source, as SolverOptions.c:249-255 keeps
+   * them (the copy does not own them) */
+  if (src->callback) o->callback = src->callback;
+  if (src->solverData) o->solverData = src->solverData;
+  if (src->solverParameters) o->solverParameters = src->solverParameters;
   return o;
 }
 

@@ -89,3 +89,47 @@ def test_local_solve_kernel(oracle, lib, sid):
                              oracle.d(x), o.p, oracle.d(st))
         np.testing.assert_array_equal(rg[3 * k:3 * k + 3], x, err_msg=f"contact {k}")
         np.testing.assert_array_equal(state[4 * k:4 * k + 4], st)
+
+
+def test_local_solve_kernel_spectral_norm_rho(oracle, lib):
+    """Rho strategy SPECTRAL_NORM (onecontact...c:132-135, 1/eig_3x3[0]):
+    the device's acos/cos may differ from the host libm's in the last bit,
+    so rho is compared to 4 ulp and the local solution to 1e-12 relative."""
+    rng = np.random.default_rng(41)
+    n = 2000
+    Ws, qs, mus = [], [], []
+    for k in range(n):
+        M = rng.normal(size=(3, 3))
+        W = M @ M.T + rng.uniform(0.1, 3) * np.eye(3)
+        if k % 4 == 0:
+            W = np.diag([2.0, 7.0, 7.0])
+        Ws.append(W.reshape(-1, order="F"))
+        qs.append(rng.normal(size=3))
+        mus.append(rng.uniform(0.1, 0.9))
+    W = np.ascontiguousarray(np.concatenate(Ws))
+    q = np.ascontiguousarray(np.concatenate(qs))
+    mu = np.array(mus)
+    o = oracle.Options.local(565)
+    o.iparam[9] = 1
+    p = local_params_from(o, 565)
+    assert p.rho_strategy == 1
+    state = np.zeros(4 * n)
+    info = np.zeros(n, dtype=np.int32)
+    rg = np.zeros(3 * n)
+    assert lib.fc3d_hip_local_solve(n, N.dptr(W), N.dptr(q), N.dptr(mu), N.dptr(rg), p,
+                                    N.dptr(state), 1, N.iptr(info)) == 0, capi.last_error()
+    L = oracle.load()
+    same = 0
+    for k in range(n):
+        st = np.zeros(4)
+        w = W[9 * k:9 * k + 9].copy()
+        assert L.oracle_local_state_init(oracle.d(w), o.p, oracle.d(st)) == 0
+        rho0 = st.copy()
+        x = np.zeros(3)
+        L.oracle_local_solve(oracle.d(w), oracle.d(q[3 * k:3 * k + 3].copy()), mu[k],
+                             oracle.d(x), o.p, oracle.d(st))
+        np.testing.assert_allclose(state[4 * k + 1:4 * k + 4], rho0[1:], rtol=4 * 2.0 ** -52)
+        np.testing.assert_allclose(rg[3 * k:3 * k + 3], x, rtol=1e-12, atol=1e-14)
+        same += int(np.array_equal(state[4 * k + 1:4 * k + 4], rho0[1:]))
+    # the diagonal blocks take the exact branch (no trigonometry)
+    assert same >= n // 4

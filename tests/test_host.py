@@ -216,3 +216,27 @@ def test_fc2d_row_builder_keeps_the_heap_intact():
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
                          timeout=300)
     assert out.returncode == 0 and out.stdout.strip().endswith("ok"), out.stderr[-2000:]
+
+
+def test_options_copy_keeps_the_pointer_links():
+    """solver_options_copy keeps callback, solverData and solverParameters
+    (SolverOptions.c:249-255)."""
+    o = N.SolverOptions(540)
+    fn = capi.CALLBACK_FN(lambda *a: None)
+    cbs = capi.Callback(None, fn)
+    o._p.contents.callback = C.pointer(cbs)
+    o._p.contents.solverData = C.c_void_p(0x1234)
+    o._p.contents.solverParameters = C.c_void_p(0x5678)
+    lib = capi.load()
+    cp = lib.solver_options_copy(o._p)
+    try:
+        assert C.addressof(cp.contents.callback.contents) == C.addressof(cbs)
+        assert cp.contents.solverData == 0x1234 and cp.contents.solverParameters == 0x5678
+    finally:
+        cp.contents.callback = None
+        cp.contents.solverData = None
+        cp.contents.solverParameters = None
+        lib.solver_options_delete(cp)
+        o._p.contents.callback = None
+        o._p.contents.solverData = None
+        o._p.contents.solverParameters = None

@@ -138,3 +138,70 @@ def test_dual_cone_projection(oracle):
         y = -x.copy()
         L.oracle_projectionOnDualCone(oracle.d(y), mu)  # P_{K*}(-x)
         np.testing.assert_allclose(pk - y, x, rtol=1e-10, atol=1e-12)
+
+
+def _eig3(oracle, A):
+    a = np.ascontiguousarray(np.asarray(A, dtype=float).reshape(-1, order="F"))
+    e = np.zeros(3)
+    oracle.load().oracle_eig_3x3(oracle.d(a), oracle.d(e))
+    return e
+
+
+def _random_symmetric(rng, kind):
+    if kind == "spd":
+        M = rng.normal(size=(3, 3))
+        return M @ M.T + 0.1 * np.eye(3)
+    if kind == "indefinite":
+        M = rng.normal(size=(3, 3))
+        return M + M.T
+    # repeated eigenvalues: Q diag(l, l, m) Q^T
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    lam = rng.uniform(0.5, 5)
+    return Q @ np.diag([lam, lam, rng.uniform(0.5, 5)]) @ Q.T
+
+
+def _eig_max_error(oracle, seed=31):
+    """Largest |eig - eigvalsh| / |A| over random symmetric matrices (SPD,
+    indefinite, repeated eigenvalues); the oracle's order is descending."""
+    rng = np.random.default_rng(seed)
+    worst = 0.0
+    for kind in ("spd", "indefinite", "repeated"):
+        for _ in range(300):
+            A = _random_symmetric(rng, kind)
+            ref = np.sort(np.linalg.eigvalsh(A))[::-1]
+            worst = max(worst, np.max(np.abs(_eig3(oracle, A) - ref)) / np.linalg.norm(A))
+    return worst
+
+
+def test_eig_3x3_against_eigvalsh(oracle):
+    """op3x3.h:1065-1111 against LAPACK's symmetric eigensolver."""
+    assert _eig_max_error(oracle) < 1e-7  # the trigonometric method loses digits
+    # near repeated eigenvalues (acos near +-1); well separated ones are tight
+    rng = np.random.default_rng(32)
+    for _ in range(200):
+        Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        lam = np.sort(rng.uniform(1, 10, size=3))[::-1] + np.array([2.0, 1.0, 0.0])
+        A = Q @ np.diag(lam) @ Q.T
+        np.testing.assert_allclose(_eig3(oracle, A), lam, rtol=1e-12)
+
+
+def test_eig_3x3_diagonal_is_returned_unsorted(oracle):
+    """The reference returns a diagonal matrix's entries in stored order, so
+    the spectral-norm rho of diag(2, 7, 7) is 1/2 (not 1/7)."""
+    np.testing.assert_array_equal(_eig3(oracle, np.diag([2.0, 7.0, 7.0])), [2.0, 7.0, 7.0])
+    rho = np.zeros(3)
+    w = np.ascontiguousarray(np.diag([2.0, 7.0, 7.0]).reshape(-1, order="F"))
+    oracle.load().oracle_rho_spectral_norm(oracle.d(w), oracle.d(rho))
+    np.testing.assert_array_equal(rho, [0.5, 0.5, 0.5])
+    A = np.array([[15.0, -1, -2], [-1, 14, 1], [-2, 1, 16]])
+    w = np.ascontiguousarray(A.reshape(-1, order="F"))
+    oracle.load().oracle_rho_spectral_norm(oracle.d(w), oracle.d(rho))
+    np.testing.assert_allclose(rho, 1.0 / np.max(np.linalg.eigvalsh(A)), rtol=1e-13)
+
+
+def test_eig_3x3_mutation_is_caught(oracle):
+    try:
+        oracle.mutation(9)
+        assert _eig_max_error(oracle) > 1e-2
+    finally:
+        oracle.mutation(0)

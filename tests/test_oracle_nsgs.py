@@ -156,3 +156,110 @@ def test_solve_many_equals_one_solve_per_problem(oracle):
         np.testing.assert_array_equal(r[3 * off:3 * (off + W.nc)], rk)
         off += W.nc
     assert ok == sum(1 for x in res if x <= 1e-6)
+
+
+@pytest.mark.parametrize("name", COLLECTION1)
+def test_spectral_norm_rho_converges(oracle, name):
+    """Reference defaults with the SPECTRAL_NORM rho strategy (local
+    iparam[9] = 1, fc3d_AlartCurnier_functions.c:814-829) on the collection."""
+    W, q, mu = load_fixture(name)
+    o = _options(oracle, 0)
+    o.internal().iparam[9] = 1
+    info, r, u, it, res = oracle.nsgs(W, q, mu, options=o)
+    assert info == 0, (name, it, res)
+    e, _ = natural_residual(W, q, mu, r)
+    assert e <= 1e-5 * (1 + 1e-6)
+
+
+FREEZE_COLLECTION = ["Capsules-i100-1090.dat", "Capsules-i100-889.dat", "Capsules-i101-404.dat",
+                     "Capsules-i103-990.dat", "Capsules-i122-1617.dat"]
+
+
+def freeze_options(oracle, variant):
+    """The five option sets of the reference test/test_nsgs_freeze_1.c:26-118
+    (run on data_collection_2.c): the test_nsgs_1.c sets with freezing 10."""
+    o = _options(oracle, variant)
+    o.iparam[15] = 10
+    return o
+
+
+@pytest.mark.parametrize("variant", range(5))
+@pytest.mark.parametrize("name", FREEZE_COLLECTION)
+def test_freeze_collection_converges(oracle, name, variant):
+    W, q, mu = load_fixture(name)
+    info, r, u, it, res = oracle.nsgs(W, q, mu, options=freeze_options(oracle, variant))
+    assert info == 0, (name, variant, it, res)
+    e, _ = natural_residual(W, q, mu, r)
+    assert e <= 1e-5 * (1 + 1e-6)
+
+
+def _freeze_replay(oracle, nfreeze=3, sweeps=60):
+    """Runs the oracle with freezing in LIGHT mode (fixed tolerance) and
+    replays fc3d_nsgs.c:719-800 from the reactions after every sweep: a
+    contact frozen by the criteria keeps its reaction bit for bit for nfreeze
+    sweeps, the light error includes its last term, and the counters reset
+    when nc - 1 are frozen.  Returns (largest relative light-error mismatch,
+    freezes seen, resets seen)."""
+    g = N.generate_random_pairs(60, 150, 0.2, 0.5, 3)
+    nc = g.nc
+    tol = 1e-3
+    o = oracle.Options()
+    o.iparam[13] = 1  # LIGHT
+    o.iparam[15] = nfreeze
+    o.iparam[0] = sweeps
+    o.dparam[0] = 1e-300
+    rec = []
+    o.set_callback(lambda r, u, err: rec.append((r, err)))
+    oracle.nsgs(g.W, g.q, g.mu, options=o)
+    o.set_callback(None)
+    frz = np.zeros(nc, dtype=int)
+    le2 = np.zeros(nc)
+    norm_r = 1e24
+    prev = np.zeros(3 * nc)
+    worst, freezes, resets = 0.0, 0, 0
+    for k, (R, err) in enumerate(rec, start=1):
+        ftmp = float(nc) * float(nc) / 1000.0
+        crit1 = 1e-300 * 1e-300 / ftmp
+        crit2 = norm_r * norm_r / ftmp
+        if np.count_nonzero(frz) >= nc - 1:
+            frz[:] = 0
+            resets += 1
+        light = 0.0
+        for c in range(nc):
+            o_, l_ = prev[3 * c:3 * c + 3], R[3 * c:3 * c + 3]
+            if frz[c] > 0:
+                assert np.array_equal(l_, o_), (k, c)
+                frz[c] -= 1
+                light += le2[c]
+                continue
+            d = [float(o_[0] - l_[0]), float(o_[1] - l_[1]), float(o_[2] - l_[2])]
+            le2[c] = d[0] * d[0] + d[1] * d[1] + d[2] * d[2]
+            light += le2[c]
+            sq = float(l_[0] * l_[0] + l_[1] * l_[1] + l_[2] * l_[2])
+            if (le2[c] <= crit1 * sq or sq <= crit2) and k >= 10:
+                frz[c] = nfreeze
+                freezes += 1
+        s = 0.0
+        for x in R:
+            s += float(x) * float(x)
+        norm_r = np.sqrt(s)
+        incr = np.sqrt(light) / norm_r
+        worst = max(worst, abs(incr - err) / max(abs(incr), 1e-300))
+        prev = R
+    del tol
+    return worst, freezes, resets, len(rec)
+
+
+def test_freeze_bookkeeping_replayed_sweep_by_sweep(oracle):
+    worst, freezes, resets, n = _freeze_replay(oracle)
+    assert n == 60 and freezes > 100 and resets > 0
+    assert worst < 1e-13
+
+
+def test_freeze_stale_light_error_mutation_is_caught(oracle):
+    try:
+        oracle.mutation(10)
+        worst, freezes, _, _ = _freeze_replay(oracle)
+        assert freezes > 100 and worst > 1e-3
+    finally:
+        oracle.mutation(0)
