@@ -55,9 +55,10 @@ typedef struct fc3d_hip_params {
   /* additions */
   int precision;             /* 0: FP64 sweeps; 1: FP32 sweeps + FP64 refinement  */
   int warm_start;            /* 0: resident reaction; 1: zero; 2: learned model,
-                                kept only if its FP64 residual after one sweep is
-                                below the zero start's; 3: the same against the
-                                resident reaction (previous time step)          */
+                                kept only if its FP64 residual after
+                                start_trial_sweeps sweeps is below the zero
+                                start's; 3: the same against the resident
+                                reaction (previous time step)                   */
   int schedule;              /* 0: autotuned; 1..10 explicit (7: single launch)   */
   int sweeps_per_launch;     /* sweeps captured per graph replay (>=1)            */
   double fp32_tol;           /* FP32 phase hands over when its light error < this */
@@ -84,6 +85,23 @@ typedef struct fc3d_hip_params {
    * 10th sweep on is skipped for this many sweeps; 0: off.  FP64 sweeps on
    * the rows of W only. */
   int freeze_contacts;
+  /* Focused passes (velocity form; no reference counterpart): after every
+   * focus_every sweeps from sweep focus_after on, the focus_count contacts
+   * whose reactions moved most in the last sweep are taken, if they carry at
+   * least focus_share of that sweep's sum |r_new - r_old|^2; every contact
+   * touching one of their bodies then gets focus_passes extra Gauss-Seidel
+   * passes (colour by colour, the sweep's own local step).  On granular
+   * piles the last hundreds of sweeps mostly move a few small hyperstatic
+   * clusters; the passes settle them.  0 passes: off (default).  The result
+   * is judged on the FP64 residual as always. */
+  int focus_passes;
+  int focus_every;           /* default 16                                         */
+  int focus_after;           /* default 64                                         */
+  int focus_count;           /* default 256                                        */
+  double focus_share;        /* default 0.5                                        */
+  /* warm_start 2, 3: sweeps each start gets before the two are compared
+   * (default 1, the first-sweep rule) */
+  int start_trial_sweeps;
 } fc3d_hip_params;
 
 typedef struct fc3d_hip_stats {
@@ -117,6 +135,12 @@ typedef struct fc3d_hip_stats {
   double sweep_tail_ms_fp32;
   int64_t sweep_tail_launches_fp32;
   double sweep_tail_bytes_fp32;
+  int start_trial_sweeps;     /* warm_start 2, 3: sweeps of the discarded start  */
+  int focus_rounds;           /* focus rounds that ran passes                    */
+  int64_t focus_listed;       /* contacts listed over those rounds               */
+  int focus_due;              /* focus rounds due (with or without passes)       */
+  double focus_last_share;    /* the last due round: share of sum |dr|^2 carried *
+                               * by its selection                                */
 } fc3d_hip_stats;
 
 FC3D_API int fc3d_hip_device_count(void);

@@ -299,7 +299,10 @@ enum SICONOS_FRICTION_3D_NSGS_HIP_DPARAM {
    * while the storage's NumericsDataVersion is unchanged and non-zero
    * (NumericsMatrix.c:112-135; the caller bumps it when values change);
    * -1 builds and frees a device problem per call. */
-  SICONOS_FRICTION_3D_NSGS_HIP_RESIDENCY = 28
+  SICONOS_FRICTION_3D_NSGS_HIP_RESIDENCY = 28,
+  /* warm start 2: sweeps each start runs before the better one is kept
+   * (0 = the default, one sweep) */
+  SICONOS_FRICTION_3D_NSGS_HIP_START_TRIAL_SWEEPS = 29
 };
 
 /* ------------------------------------------------------------------------ */

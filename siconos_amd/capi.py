@@ -94,7 +94,9 @@ class HipParams(C.Structure):
                 ("fp32_local_tol", C.c_double), ("instrument", C.c_int),
                 ("accel_depth", C.c_int), ("accel_every", C.c_int), ("accel_reg", C.c_double),
                 ("velocity_form", C.c_int), ("fp32_local_solver", C.c_int),
-                ("freeze_contacts", C.c_int)]
+                ("freeze_contacts", C.c_int), ("focus_passes", C.c_int),
+                ("focus_every", C.c_int), ("focus_after", C.c_int), ("focus_count", C.c_int),
+                ("focus_share", C.c_double), ("start_trial_sweeps", C.c_int)]
 
 
 class HipStats(C.Structure):
@@ -109,7 +111,10 @@ class HipStats(C.Structure):
                 ("sweep_kernel_launches_fp32", C.c_int64),
                 ("sweep_kernel_bytes_fp32", C.c_double),
                 ("sweep_tail_ms_fp32", C.c_double), ("sweep_tail_launches_fp32", C.c_int64),
-                ("sweep_tail_bytes_fp32", C.c_double)]
+                ("sweep_tail_bytes_fp32", C.c_double), ("start_trial_sweeps", C.c_int),
+                ("focus_rounds", C.c_int),
+                ("focus_listed", C.c_int64), ("focus_due", C.c_int),
+                ("focus_last_share", C.c_double)]
 
 
 class Fc2dParams(C.Structure):

@@ -1,0 +1,434 @@
+// fc3d_focus.hpp — focused passes of the velocity-form sweep (no reference
+// counterpart; enabled by fc3d_hip_params.focus_passes).
+//
+// On granular piles the coloured Gauss-Seidel sweep reaches a residual near
+// the tolerance after a few hundred sweeps and then spends most of its sweeps
+// on a handful of small hyperstatic clusters (a few bodies carrying many
+// contacts), whose reactions drift along near-null directions of W: on a
+// 1M-contact pile, after 300 sweeps the top 0.1% of contacts carry 90% of
+// sum |r_new - r_old|^2.  A focus round gives those clusters extra
+// Gauss-Seidel passes of their own:
+//   1. the sweep that ends a focus interval stores r_new - r_old per contact
+//      (BodyPtrs::fdiff);
+//   2. k_focus_hist bins |r_new - r_old|^2 by the top bits of its FP32
+//      representation, and k_focus_threshold picks the bin at which the
+//      largest DevParams::focus_count contacts are reached; the round is
+//      skipped unless those carry focus_share of the sum (a spread-out error
+//      is the sweep's own work);
+//   3. k_focus_mark marks the bodies of the selected contacts and
+//      k_focus_collect lists, per colour, every contact touching a marked
+//      body (the clusters and their first ring);
+//   4. k_focus_passes runs `passes` Gauss-Seidel passes over the list in one
+//      workgroup, colour after colour, with the sweep's own per-contact step.
+// k_focus_clear, k_focus_hist and k_focus_threshold test whether a round is
+// due; the threshold kernel records the round in SolveState::focus_last and
+// sets FocusState::active, which the later kernels follow.
+// Within a colour the list order comes from atomics, but a colour's contacts
+// share no body, so the passes' results do not depend on it.  A round whose
+// lists overflow their capacity is skipped whole, so the selection is
+// deterministic too.  The result is still judged on the FP64 residual.
+#pragma once
+#include "fc3d_velocity.hpp"
+
+namespace fc3d {
+
+constexpr int FOCUS_BINS = 2048;  // top 11 bits of a non-negative float (sign bit 0)
+constexpr int FOCUS_FINITE = 0x7f800000 >> 20;  // first key of +inf and NaN
+
+struct FocusState {
+  unsigned hist[FOCUS_BINS];
+  int threshold;  // key of the selection: contacts with key >= threshold
+  int round;      // mark of this round (grows across solves; bodies hold the last mark)
+  int active;     // 1: this round selects, lists and runs its passes
+  int rounds;     // rounds with passes in this solve
+  int listed_sum; // contacts listed over those rounds
+  int due;        // rounds due in this solve
+  int pad;
+  double last_share;  // the last due round: share of sum |dr|^2 its selection carried
+  double last_total;  // ... and that sum
+};
+
+// At solve start: the per-solve statistics (the round mark keeps growing).
+__global__ void k_focus_reset(FocusState* fs) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    fs->active = 0;
+    fs->rounds = 0;
+    fs->listed_sum = 0;
+    fs->due = 0;
+    fs->last_share = 0.0;
+    fs->last_total = 0.0;
+  }
+}
+
+__device__ __forceinline__ int focus_key(float d2) { return (int)(__float_as_uint(d2) >> 20); }
+
+// A round is due once focus_every sweeps have passed since the last one
+// (SolveState::focus_last), from sweep focus_after on, unless the solve
+// stopped with the last sweep.  Rounds are enqueued at the end of each graph
+// replay; k_focus_threshold decides and records, the later kernels follow
+// FocusState::active.
+__device__ __forceinline__ bool focus_round_due(const DevParams* prm, const SolveState* st,
+                                                int want_phase) {
+  return !st->stop && st->phase == want_phase && st->iter >= prm->focus_after &&
+         st->iter - st->focus_last >= prm->focus_every;
+}
+
+__global__ void k_focus_clear(const DevParams* prm, const SolveState* st, FocusState* fs,
+                              int* cnt, int ncolours, int want_phase) {
+  // (runs before k_focus_threshold, so it still sees the round as due)
+  if (!focus_round_due(prm, st, want_phase)) return;
+  for (int b = threadIdx.x; b < FOCUS_BINS; b += blockDim.x) fs->hist[b] = 0u;
+  for (int c = threadIdx.x; c < ncolours; c += blockDim.x) cnt[c] = 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_focus_hist(const DevParams* prm, const SolveState* st,
+                                                     FocusState* fs, const int* __restrict__ len,
+                                                     const typename vec4<T>::type* __restrict__ d,
+                                                     int nslots, int want_phase) {
+  __shared__ unsigned h[FOCUS_BINS];
+  if (!focus_round_due(prm, st, want_phase)) return;
+  for (int b = threadIdx.x; b < FOCUS_BINS; b += blockDim.x) h[b] = 0u;
+  __syncthreads();
+  for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < nslots;
+       slot += gridDim.x * blockDim.x) {
+    if (len[slot] < 0) continue;
+    const typename vec4<T>::type v = d[slot];
+    const float d2 = float(v.x * v.x + v.y * v.y + v.z * v.z);
+    atomicAdd(&h[focus_key(d2)], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < FOCUS_BINS; b += blockDim.x)
+    if (h[b]) atomicAdd(&fs->hist[b], h[b]);
+}
+
+// The threshold bin and whether the round runs, in one workgroup of 1024
+// threads: the bins' counts and masses (count times the bin's smallest value)
+// go to LDS, suffix sums are taken by a block scan, and the highest bin at
+// which the suffix count reaches focus_count is the threshold.  Keys from
+// FOCUS_FINITE on hold infinities and NaNs, which no selection takes.
+constexpr int FOCUS_SCAN = 1024;
+__global__ __launch_bounds__(FOCUS_SCAN) void k_focus_threshold(const DevParams* prm,
+                                                                SolveState* st, FocusState* fs,
+                                                                int want_phase) {
+  __shared__ long cnt[FOCUS_SCAN];
+  __shared__ double mass[FOCUS_SCAN];
+  __shared__ int thr_sh;
+  const int t = threadIdx.x;
+  const bool due = focus_round_due(prm, st, want_phase);
+  __syncthreads();  // every thread has read the state before thread 0 updates it
+  if (!due) {
+    if (t == 0) fs->active = 0;
+    return;
+  }
+  // thread t owns bins 2t and 2t + 1; the scan runs from the top bin down,
+  // so it is taken over the reversed thread order
+  const int u = FOCUS_SCAN - 1 - t;
+  long c = 0;
+  double m = 0.0;
+  for (int b = 2 * u; b < 2 * u + 2; b++) {
+    if (b > 0 && b < FOCUS_FINITE) {
+      const unsigned h = fs->hist[b];
+      c += h;
+      m += (double)h * (double)__uint_as_float((unsigned)b << 20);
+    }
+  }
+  cnt[t] = c;
+  mass[t] = m;
+  if (t == 0) thr_sh = -1;
+  __syncthreads();
+  for (int off = 1; off < FOCUS_SCAN; off <<= 1) {  // inclusive scan over t
+    const long ca = t >= off ? cnt[t - off] : 0;
+    const double ma = t >= off ? mass[t - off] : 0.0;
+    __syncthreads();
+    cnt[t] += ca;
+    mass[t] += ma;
+    __syncthreads();
+  }
+  // bins above thread t's pair: the scan value of thread t - 1
+  const long above = t > 0 ? cnt[t - 1] : 0;
+  const long hi = fs->hist[2 * u + 1] * (long)(2 * u + 1 < FOCUS_FINITE);
+  const int K = prm->focus_count;
+  if (above < K && above + hi >= K) thr_sh = 2 * u + 1;
+  else if (above + hi < K && cnt[t] >= K && 2 * u > 0) thr_sh = 2 * u;
+  __syncthreads();
+  if (t != 0) return;
+  st->focus_last = st->iter;
+  const int thr = thr_sh;
+  const double total = mass[FOCUS_SCAN - 1];
+  double top = 0.0;
+  if (thr > 0) {
+    // mass of the bins >= thr: the scan through the pair holding thr, less
+    // the pair's lower bin when thr is its upper one
+    const int tt = FOCUS_SCAN - 1 - thr / 2;
+    top = mass[tt];
+    if (thr & 1) {
+      const int b = thr - 1;
+      if (b > 0) top -= (double)fs->hist[b] * (double)__uint_as_float((unsigned)b << 20);
+    }
+  }
+  fs->round += 1;
+  fs->due += 1;
+  fs->last_share = total > 0.0 ? top / total : 0.0;
+  fs->last_total = total;
+  fs->threshold = thr;
+  fs->active = thr > 0 && total > 0.0 && top >= prm->focus_share * total;
+}
+
+__global__ void k_focus_mark(const FocusState* fs,
+                             const int* __restrict__ len, const int* __restrict__ ba,
+                             const int* __restrict__ bb,
+                             const float4* __restrict__ d32, const double4* __restrict__ d64,
+                             int* __restrict__ bmark, int nslots) {
+  if (!fs->active) return;
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= nslots || len[slot] < 0) return;
+  float d2;
+  if (d32) {
+    const float4 v = d32[slot];
+    d2 = v.x * v.x + v.y * v.y + v.z * v.z;
+  } else {
+    const double4 v = d64[slot];
+    d2 = float(v.x * v.x + v.y * v.y + v.z * v.z);
+  }
+  const int key = focus_key(d2);
+  if (key < fs->threshold || key >= FOCUS_FINITE) return;
+  const int r = fs->round;
+  if (ba[slot] >= 0) bmark[ba[slot]] = r;
+  if (bb[slot] >= 0) bmark[bb[slot]] = r;
+}
+
+__global__ void k_focus_collect(const FocusState* fs, const int* __restrict__ len, const int* __restrict__ ba,
+                                const int* __restrict__ bb, const int* __restrict__ bmark,
+                                const int* __restrict__ slice_colour, int* cnt, int* list,
+                                int capc, int nslots) {
+  if (!fs->active) return;
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= nslots || len[slot] < 0) return;
+  const int r = fs->round;
+  const bool hot = (ba[slot] >= 0 && bmark[ba[slot]] == r) || (bb[slot] >= 0 && bmark[bb[slot]] == r);
+  if (!hot) return;
+  const int c = slice_colour[slot >> 6];
+  const int pos = atomicAdd(&cnt[c], 1);
+  if (pos < capc) list[(size_t)c * capc + pos] = slot;
+}
+
+// Staged passes (diagonal M^-1): the listed contacts' data are copied once
+// per round into a compact buffer, entry i of colour c at off[c] + j, each
+// field a plane of N entries, so that a pass reads coalesced lines that stay
+// in L2; the reactions live there during the passes and go back at the end.
+template <typename T>
+struct FocusStage {
+  int* slot;   // [N] the entry's slot
+  int* ba;     // [N]
+  int* bb;     // [N]
+  T* H;        // [36][N]
+  T* W;        // [9][N]
+  T* mu;       // [N]
+  T* minv;     // [12][N]: side a 0-5, side b 6-11 (diagonal M^-1)
+  typename vec4<T>::type* r;  // [N]
+  T* pli;      // [N] (null when the local solver keeps no PLI state)
+  int N;
+};
+
+// off[c]: first entry of colour c (the list counts scanned); -1 in off[0]
+// marks an overflow (the round is then skipped).
+__device__ __forceinline__ void focus_offsets(const int* cnt, int ncolours, int capc, int* off) {
+  int acc = 0;
+  for (int c = 0; c < ncolours; c++) {
+    off[c] = acc;
+    if (cnt[c] > capc) {
+      off[0] = -1;
+      return;
+    }
+    acc += cnt[c];
+  }
+  off[ncolours] = acc;
+}
+
+constexpr int FOCUS_MAX_COLOURS = 64;
+
+template <typename T>
+__global__ void k_focus_stage(const FocusState* fs, SweepPtrs<T> P, BodyPtrs<T> B,
+                              const int* __restrict__ cnt, const int* __restrict__ list, int capc,
+                              int ncolours, FocusStage<T> S) {
+  if (!fs->active) return;
+  __shared__ int off[FOCUS_MAX_COLOURS + 1];
+  if (threadIdx.x == 0) focus_offsets(cnt, ncolours, capc, off);
+  __syncthreads();
+  if (off[0] < 0) return;
+  const int total = off[ncolours];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    int c = 0;
+    while (off[c + 1] <= i) c++;
+    const int slot = list[(size_t)c * capc + (i - off[c])];
+    const int s = slot >> 6, lane = slot & 63;
+    S.slot[i] = slot;
+    const int a = B.ba[slot], b = B.bb[slot];
+    S.ba[i] = a;
+    S.bb[i] = b;
+#pragma unroll 4
+    for (int e = 0; e < 36; e++) S.H[(size_t)e * S.N + i] = B.H[((size_t)s * 36 + e) * 64 + lane];
+#pragma unroll
+    for (int e = 0; e < 9; e++) S.W[(size_t)e * S.N + i] = P.diag[((size_t)s * 9 + e) * 64 + lane];
+    S.mu[i] = P.mu[slot];
+#pragma unroll
+    for (int l = 0; l < 6; l++) {
+      S.minv[(size_t)l * S.N + i] = a >= 0 ? B.minv[6 * (size_t)a + l] : T(0);
+      S.minv[(size_t)(6 + l) * S.N + i] = b >= 0 ? B.minv[6 * (size_t)b + l] : T(0);
+    }
+    S.r[i] = P.r[slot];
+    if (S.pli) S.pli[i] = P.pli[slot];
+  }
+}
+
+template <typename T>
+__global__ void k_focus_unstage(const FocusState* fs, SweepPtrs<T> P, const int* __restrict__ cnt,
+                                int capc, int ncolours, FocusStage<T> S) {
+  if (!fs->active) return;
+  __shared__ int off[FOCUS_MAX_COLOURS + 1];
+  if (threadIdx.x == 0) focus_offsets(cnt, ncolours, capc, off);
+  __syncthreads();
+  if (off[0] < 0) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < off[ncolours];
+       i += gridDim.x * blockDim.x) {
+    const int slot = S.slot[i];
+    P.r[slot] = S.r[i];
+    if (S.pli) P.pli[slot] = S.pli[i];
+  }
+}
+
+// The velocity-form step of staged entry i: the arithmetic of vel_slot with
+// the entry's data read from the stage (the reaction is updated there).
+template <typename T>
+__device__ __forceinline__ void focus_step(const FocusStage<T>& S, const BodyPtrs<T>& B,
+                                           const StepConsts& sc, T tol, int i) {
+  const size_t N = (size_t)S.N;
+  SlotInputs<T> in;
+#pragma unroll
+  for (int e = 0; e < 9; e++) in.lp.W[e] = S.W[e * N + i];
+  in.lp.mu = S.mu[i];
+  in.ro = S.r[i];
+  in.pli = S.pli ? S.pli[i] : T(0);
+  const int kb[2] = {S.ba[i], S.bb[i]};
+  T h[36];
+#pragma unroll
+  for (int e = 0; e < 36; e++) h[e] = S.H[e * N + i];
+  typename vec4<T>::type va[2][2];
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    const size_t k = kb[side] >= 0 ? (size_t)kb[side] : 0;
+    va[side][0] = B.V[2 * k];
+    va[side][1] = B.V[2 * k + 1];
+  }
+  T u[3] = {T(0), T(0), T(0)};
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    if (kb[side] < 0) continue;
+    const T v[6] = {va[side][0].x, va[side][0].y, va[side][0].z,
+                    va[side][1].x, va[side][1].y, va[side][1].z};
+#pragma unroll
+    for (int l = 0; l < 6; l++)
+#pragma unroll
+      for (int e = 0; e < 3; e++) u[e] += h[18 * side + e + 3 * l] * v[l];
+  }
+  const T rold[3] = {in.ro.x, in.ro.y, in.ro.z};
+  T wr[3] = {T(0), T(0), T(0)};
+  mvp3x3<T>(in.lp.W, rold, wr);
+#pragma unroll
+  for (int e = 0; e < 3; e++) in.ql[e] = u[e] - wr[e];
+  // finish_slot on the stage: its slot index is the entry's
+  SweepPtrs<T> Pc;
+  Pc.r = S.r;
+  Pc.pli = S.pli;
+  Pc.f = nullptr;
+  typename vec4<T>::type out;
+  double light, nr2;
+  finish_slot<T>(Pc, sc, i >> 6, i & 63, tol, in, &light, &nr2, &out);
+  const T dr[3] = {out.x - rold[0], out.y - rold[1], out.z - rold[2]};
+  typename vec4<T>::type vn[2][2];
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    if (kb[side] < 0) continue;
+    T dv[6];
+#pragma unroll
+    for (int l = 0; l < 6; l++) {
+      const T g = h[18 * side + 3 * l] * dr[0] + h[18 * side + 3 * l + 1] * dr[1] +
+                  h[18 * side + 3 * l + 2] * dr[2];
+      dv[l] = S.minv[(6 * side + l) * N + i] * g;
+    }
+    const bool again = side == 1 && kb[1] == kb[0];
+    typename vec4<T>::type v0 = again ? vn[0][0] : va[side][0];
+    typename vec4<T>::type v1 = again ? vn[0][1] : va[side][1];
+    v0.x += dv[0]; v0.y += dv[1]; v0.z += dv[2];
+    v1.x += dv[3]; v1.y += dv[4]; v1.z += dv[5];
+    vn[side][0] = v0;
+    vn[side][1] = v1;
+    B.V[2 * (size_t)kb[side]] = v0;
+    B.V[2 * (size_t)kb[side] + 1] = v1;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(vel_tail_threads<T>()) void k_focus_passes_staged(
+    FocusStage<T> S, BodyPtrs<T> B, const DevParams* __restrict__ prm, const SolveState* st,
+    FocusState* fs, const int* __restrict__ cnt, int capc, int ncolours, int passes,
+    int want_phase) {
+  constexpr int nt = vel_tail_threads<T>();
+  __shared__ int off[FOCUS_MAX_COLOURS + 1];
+  if (!fs->active) return;
+  if (threadIdx.x == 0) focus_offsets(cnt, ncolours, capc, off);
+  __syncthreads();
+  if (off[0] < 0 || off[ncolours] == 0) return;
+  const StepConsts sc = step_consts<T>(prm);
+  const T tol = T(want_phase ? prm->fp32_local_tol : st->local_tol);
+  for (int pass = 0; pass < passes; pass++) {
+    for (int c = 0; c < ncolours; c++) {
+      const int b = off[c], e = off[c + 1];
+      if (b == e) continue;  // uniform
+      for (int i = b + (int)threadIdx.x; i < e; i += nt) focus_step<T>(S, B, sc, tol, i);
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x == 0) {
+    fs->rounds += 1;
+    fs->listed_sum += off[ncolours];
+  }
+}
+
+// The passes, in one workgroup: colour after colour, a workgroup barrier
+// between colours (the next colour reads this colour's body velocities).
+// The workgroup is as large as k_sweep_vel's registers allow on one CU.
+template <typename T>
+__global__ __launch_bounds__(vel_tail_threads<T>()) void k_focus_passes(
+    SweepPtrs<T> P, BodyPtrs<T> B, const DevParams* __restrict__ prm, const SolveState* st,
+    FocusState* fs, const int* __restrict__ cnt, const int* __restrict__ list, int capc,
+    int ncolours, int passes, int want_phase) {
+  constexpr int nt = vel_tail_threads<T>();
+  if (!fs->active) return;
+  int listed = 0;
+  for (int c = 0; c < ncolours; c++) {
+    if (cnt[c] > capc) return;  // overflow: the round is skipped whole
+    listed += cnt[c];
+  }
+  if (listed == 0) return;
+  for (int pass = 0; pass < passes; pass++) {
+    for (int c = 0; c < ncolours; c++) {
+      const int n = cnt[c];
+      if (n == 0) continue;  // uniform: every thread skips the empty colour
+      const int* lc = list + (size_t)c * capc;
+      for (int i = threadIdx.x; i < n; i += nt) {
+        const int slot = lc[i];
+        double light, nr2;
+        vel_slot<T, false, false>(P, B, prm, st, slot >> 6, slot & 63, want_phase, light, nr2);
+      }
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x == 0) {
+    fs->rounds += 1;
+    fs->listed_sum += listed;
+  }
+}
+
+}  // namespace fc3d

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include <hip/hip_ext.h>
@@ -27,6 +28,7 @@
 #include "fc3d_internal.hpp"
 #include "fc3d_accel.hpp"
 #include "fc3d_velocity.hpp"
+#include "fc3d_focus.hpp"
 #include "fc3d_layout.hpp"
 #include "fc3d_mlp.hpp"
 #include "fc3d_sweep.hpp"
@@ -559,6 +561,18 @@ struct fc3d_hip_problem {
   int* d_frz = nullptr;
   double* d_le2 = nullptr;
   int* d_fcnt = nullptr;
+  // focused passes (velocity form, allocated on first use)
+  int focus_passes = 0, graph_focus[2] = {0, 0};
+  int focus_capc = 0;
+  FocusState* d_fs = nullptr;
+  int *d_bmark = nullptr, *d_slice_colour = nullptr, *d_focus_cnt = nullptr,
+      *d_focus_list = nullptr;
+  float4* d_fdiff32 = nullptr;
+  double4* d_fdiff64 = nullptr;
+  // staged passes (diagonal M^-1, at most FOCUS_MAX_COLOURS colours)
+  int focus_staged = 0;
+  FocusStage<float> stage32 = {};
+  FocusStage<double> stage64 = {};
 };
 
 namespace fc3d {
@@ -600,6 +614,7 @@ static BodyPtrs<double> bodies64(fc3d_hip_problem* h) {
   B.ba = h->d_ba; B.bb = h->d_bb; B.H = h->d_H; B.F = h->d_F; B.minv = h->d_minv;
   B.minv_diag = h->minv_diag;
   B.V = h->d_V; B.vfree = h->d_vfree; B.inc_ptr = h->d_inc_ptr; B.inc = h->d_inc;
+  if (h->focus_passes > 0) B.fdiff = h->d_fdiff64;
   return B;
 }
 static BodyPtrs<float> bodies32(fc3d_hip_problem* h) {
@@ -607,7 +622,128 @@ static BodyPtrs<float> bodies32(fc3d_hip_problem* h) {
   B.ba = h->d_ba; B.bb = h->d_bb; B.H = h->d_H32; B.F = h->d_F32; B.minv = h->d_minv32;
   B.minv_diag = h->minv_diag; B.V = h->d_V32; B.vfree = h->d_vfree; B.inc_ptr = h->d_inc_ptr;
   B.inc = h->d_inc;
+  if (h->focus_passes > 0) B.fdiff = h->d_fdiff32;
   return B;
+}
+
+// Focused passes: buffers on first use.  Each colour's list holds up to
+// focus_capc contacts (a round that lists more is skipped).
+static int ensure_focus(fc3d_hip_problem* h, int mixed) {
+  int rc;
+  if (!h->d_fs) {
+    h->focus_capc = 4096;
+    std::vector<int> sc((size_t)h->nslices);
+    for (int c = 0; c < h->ncolours; c++)
+      for (int sl = h->colour_slice_ptr[(size_t)c]; sl < h->colour_slice_ptr[(size_t)c + 1]; sl++)
+        sc[(size_t)sl] = c;
+    if ((rc = dev_alloc(&h->d_fs, 1)) || (rc = dev_alloc(&h->d_bmark, (size_t)h->nbodies)) ||
+        (rc = dev_upload(&h->d_slice_colour, sc)) ||
+        (rc = dev_alloc(&h->d_focus_cnt, (size_t)h->ncolours)) ||
+        (rc = dev_alloc(&h->d_focus_list, (size_t)h->focus_capc * (size_t)h->ncolours)) ||
+        (rc = dev_alloc(&h->d_fdiff64, (size_t)h->nslots)))
+      return rc;
+    FC3D_HIP_TRY(hipMemsetAsync(h->d_fs, 0, sizeof(FocusState), h->stream));
+    FC3D_HIP_TRY(hipMemsetAsync(h->d_bmark, 0, sizeof(int) * (size_t)h->nbodies, h->stream));
+  }
+  if (mixed && !h->d_fdiff32 && (rc = dev_alloc(&h->d_fdiff32, (size_t)h->nslots))) return rc;
+  h->focus_staged = h->minv_diag && h->ncolours <= FOCUS_MAX_COLOURS;
+  if (h->focus_staged) {
+    const int N = h->focus_capc * h->ncolours;
+    auto alloc_stage = [&](auto& S, auto* vt) -> int {
+      using TT = std::remove_pointer_t<decltype(vt)>;
+      if (S.slot) return 0;
+      S.N = N;
+      int r;
+      if ((r = dev_alloc(&S.slot, (size_t)N)) || (r = dev_alloc(&S.ba, (size_t)N)) ||
+          (r = dev_alloc(&S.bb, (size_t)N)) || (r = dev_alloc(&S.H, 36 * (size_t)N)) ||
+          (r = dev_alloc(&S.W, 9 * (size_t)N)) || (r = dev_alloc(&S.mu, (size_t)N)) ||
+          (r = dev_alloc(&S.minv, 12 * (size_t)N)) || (r = dev_alloc(&S.r, (size_t)N)) ||
+          (r = dev_alloc(&S.pli, (size_t)N)))
+        return r;
+      (void)sizeof(TT);
+      return 0;
+    };
+    if ((rc = alloc_stage(h->stage64, (double*)nullptr))) return rc;
+    if (mixed && (rc = alloc_stage(h->stage32, (float*)nullptr))) return rc;
+  }
+  return 0;
+}
+
+static void free_stage_bufs(fc3d_hip_problem* h) {
+  void* p[] = {h->stage64.slot, h->stage64.ba, h->stage64.bb, h->stage64.H, h->stage64.W,
+               h->stage64.mu, h->stage64.minv, h->stage64.r, h->stage64.pli,
+               h->stage32.slot, h->stage32.ba, h->stage32.bb, h->stage32.H, h->stage32.W,
+               h->stage32.mu, h->stage32.minv, h->stage32.r, h->stage32.pli};
+  for (void* q : p)
+    if (q) hipFree(q);
+}
+
+// A focus round after the sweeps just enqueued (fc3d_focus.hpp); every kernel
+// returns at once unless a round is due.
+static void enqueue_focus(fc3d_hip_problem* h, int phase) {
+  const unsigned nb = (unsigned)((h->nslots + 255) / 256);
+  const int nbh = std::max(1, std::min(256, (h->nslots + 1023) / 1024));
+  hipLaunchKernelGGL(k_focus_clear, dim3(1), dim3(256), 0, h->stream, h->prm, h->st, h->d_fs,
+                     h->d_focus_cnt, h->ncolours, phase);
+  if (phase)
+    hipLaunchKernelGGL(k_focus_hist<float>, dim3(nbh), dim3(1024), 0, h->stream, h->prm, h->st,
+                       h->d_fs, h->len, h->d_fdiff32, h->nslots, phase);
+  else
+    hipLaunchKernelGGL(k_focus_hist<double>, dim3(nbh), dim3(1024), 0, h->stream, h->prm, h->st,
+                       h->d_fs, h->len, h->d_fdiff64, h->nslots, phase);
+  hipLaunchKernelGGL(k_focus_threshold, dim3(1), dim3(FOCUS_SCAN), 0, h->stream, h->prm, h->st,
+                     h->d_fs, phase);
+  hipLaunchKernelGGL(k_focus_mark, dim3(nb), dim3(256), 0, h->stream, h->d_fs, h->len, h->d_ba,
+                     h->d_bb, phase ? h->d_fdiff32 : nullptr, phase ? nullptr : h->d_fdiff64,
+                     h->d_bmark, h->nslots);
+  hipLaunchKernelGGL(k_focus_collect, dim3(nb), dim3(256), 0, h->stream, h->d_fs, h->len,
+                     h->d_ba, h->d_bb, h->d_bmark, h->d_slice_colour, h->d_focus_cnt,
+                     h->d_focus_list, h->focus_capc, h->nslots);
+  if (h->focus_staged) {
+    const dim3 gs(64), bs(256);
+    if (phase) {
+      BodyPtrs<float> B = bodies32(h);
+      B.fdiff = nullptr;
+      FocusStage<float> S = h->stage32;
+      if (h->pli32_off) S.pli = nullptr;
+      const SweepPtrs<float> P = ptrs32(h);
+      hipLaunchKernelGGL(k_focus_stage<float>, gs, bs, 0, h->stream, h->d_fs, P, B,
+                         h->d_focus_cnt, h->d_focus_list, h->focus_capc, h->ncolours, S);
+      hipLaunchKernelGGL(k_focus_passes_staged<float>, dim3(1), dim3(vel_tail_threads<float>()),
+                         0, h->stream, S, B, h->prm, h->st, h->d_fs, h->d_focus_cnt,
+                         h->focus_capc, h->ncolours, h->focus_passes, phase);
+      hipLaunchKernelGGL(k_focus_unstage<float>, gs, bs, 0, h->stream, h->d_fs, P,
+                         h->d_focus_cnt, h->focus_capc, h->ncolours, S);
+    } else {
+      BodyPtrs<double> B = bodies64(h);
+      B.fdiff = nullptr;
+      const SweepPtrs<double> P = ptrs64(h, h->part);
+      hipLaunchKernelGGL(k_focus_stage<double>, gs, bs, 0, h->stream, h->d_fs, P, B,
+                         h->d_focus_cnt, h->d_focus_list, h->focus_capc, h->ncolours,
+                         h->stage64);
+      hipLaunchKernelGGL(k_focus_passes_staged<double>, dim3(1),
+                         dim3(vel_tail_threads<double>()), 0, h->stream, h->stage64, B, h->prm,
+                         h->st, h->d_fs, h->d_focus_cnt, h->focus_capc, h->ncolours,
+                         h->focus_passes, phase);
+      hipLaunchKernelGGL(k_focus_unstage<double>, gs, bs, 0, h->stream, h->d_fs, P,
+                         h->d_focus_cnt, h->focus_capc, h->ncolours, h->stage64);
+    }
+    return;
+  }
+  if (phase) {
+    BodyPtrs<float> B = bodies32(h);
+    B.fdiff = nullptr;
+    hipLaunchKernelGGL(k_focus_passes<float>, dim3(1), dim3(vel_tail_threads<float>()), 0,
+                       h->stream, ptrs32(h), B, h->prm, h->st, h->d_fs, h->d_focus_cnt,
+                       h->d_focus_list, h->focus_capc, h->ncolours, h->focus_passes, phase);
+  } else {
+    BodyPtrs<double> B = bodies64(h);
+    B.fdiff = nullptr;
+    hipLaunchKernelGGL(k_focus_passes<double>, dim3(1), dim3(vel_tail_threads<double>()), 0,
+                       h->stream, ptrs64(h, h->part), B, h->prm, h->st, h->d_fs,
+                       h->d_focus_cnt, h->d_focus_list, h->focus_capc, h->ncolours,
+                       h->focus_passes, phase);
+  }
 }
 
 static int ensure_fp32(fc3d_hip_problem* h) {
@@ -1077,7 +1213,7 @@ static int build_graph(fc3d_hip_problem* h, int phase, int sweeps, int schedule)
   if (h->graph[phase] && h->graph_sweeps[phase] == sweeps && h->graph_schedule[phase] == schedule &&
       h->graph_accel[phase] == h->accel_on && h->graph_vel[phase] == h->vel_on &&
       h->graph_tail[phase] == vel_tail_begin(h, phase) && h->graph_pli32_off[phase] == h->pli32_off &&
-      h->graph_freeze[phase] == h->freeze)
+      h->graph_freeze[phase] == h->freeze && h->graph_focus[phase] == h->focus_passes)
     return 0;
   if (h->graph[phase]) {
     hipGraphExecDestroy(h->graph[phase]);
@@ -1086,6 +1222,7 @@ static int build_graph(fc3d_hip_problem* h, int phase, int sweeps, int schedule)
   hipGraph_t g = nullptr;
   FC3D_HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   for (int k = 0; k < sweeps; k++) enqueue_sweep(h, phase, schedule, nullptr, 0);
+  if (h->focus_passes > 0) enqueue_focus(h, phase);
   FC3D_HIP_TRY(hipStreamEndCapture(h->stream, &g));
   FC3D_HIP_TRY(hipGraphInstantiate(&h->graph[phase], g, nullptr, nullptr, 0));
   hipGraphDestroy(g);
@@ -1096,6 +1233,7 @@ static int build_graph(fc3d_hip_problem* h, int phase, int sweeps, int schedule)
   h->graph_tail[phase] = vel_tail_begin(h, phase);
   h->graph_pli32_off[phase] = h->pli32_off;
   h->graph_freeze[phase] = h->freeze;
+  h->graph_focus[phase] = h->focus_passes;
   return 0;
 }
 
@@ -1115,8 +1253,14 @@ static int final_residual(fc3d_hip_problem* h) {
 // smaller residual (the fallback on a tie or a non-finite residual).  The kept
 // sweep counts as the solve's first; the other one is extra work inside the
 // solve's time.
-static int learned_start(fc3d_hip_problem* h, int phase, int schedule, int max_iter, int previous,
-                         SolveState* s0, fc3d_hip_stats* out) {
+// The learned warm start (warm_start 2, 3): the network's start and the
+// fallback (zero, or the resident reaction) each get `trial` full sweeps of
+// the solve's own kind (rows of W or velocity form, this phase's
+// precision), and the one with the lower FP64 residual after them is kept,
+// with its solve state, so its sweeps count; the other leg's sweeps are
+// reported apart.  The zero start wins ties and non-finite residuals.
+static int learned_start(fc3d_hip_problem* h, int phase, int schedule, int trial_sweeps,
+                         int previous, SolveState* s0, fc3d_hip_stats* out) {
   const size_t ns = (size_t)h->nslots;
   const unsigned nb = (unsigned)((ns + 255) / 256);
   int rc;
@@ -1133,7 +1277,9 @@ static int learned_start(fc3d_hip_problem* h, int phase, int schedule, int max_i
                      h->feat, (float*)nullptr);
   if ((rc = mlp_forward_device(h->model, h->nslots, h->feat, h->featX, h->ypred, h->stream)))
     return rc;
+  const int m = trial_sweeps > 0 ? trial_sweeps : 1;
   double res[2];
+  SolveState after[2];
   for (int trial = 0; trial < 2; trial++) {
     if (trial == 0 && previous)
       FC3D_HIP_TRY(hipMemcpyAsync(h->r, h->x, sizeof(double4) * ns, hipMemcpyDeviceToDevice,
@@ -1145,23 +1291,20 @@ static int learned_start(fc3d_hip_problem* h, int phase, int schedule, int max_i
       hipLaunchKernelGGL(k_apply_start, dim3(nb), dim3(256), 0, h->stream, ptrs64(h, h->part),
                          h->nslots, (const float*)h->ypred);
     hipLaunchKernelGGL(k_fill<double>, dim3(nb), dim3(256), 0, h->stream, h->pli, 1.0, ns);
-    if (!s0->stop) {
-      if (phase)
-        hipLaunchKernelGGL(k_r64_to_r32, dim3(nb), dim3(256), 0, h->stream, h->r, h->r32, h->pli,
-                           h->pli32, ns);
-      for (int c = 0; c < h->ncolours; c++) {
-        const int sb = h->colour_slice_ptr[(size_t)c], se = h->colour_slice_ptr[(size_t)c + 1];
-        launch_sweep(h, phase, schedule, (se - sb + 3) / 4, sb, se);
-      }
-      if (phase)
-        hipLaunchKernelGGL(k_r32_to_r64, dim3(nb), dim3(256), 0, h->stream, h->r32, h->r,
-                           h->pli32, h->pli, ns);
-    }
+    FC3D_HIP_TRY(hipMemcpyAsync(h->st, s0, sizeof *s0, hipMemcpyHostToDevice, h->stream));
+    if (phase)
+      hipLaunchKernelGGL(k_r64_to_r32, dim3(nb), dim3(256), 0, h->stream, h->r, h->r32, h->pli,
+                         h->pli32, ns);
+    if (h->vel_on) launch_vinit(h, phase, nullptr, nullptr);
+    for (int k = 0; k < m; k++) enqueue_sweep(h, phase, schedule, nullptr, 0);
+    if (phase)
+      hipLaunchKernelGGL(k_r32_to_r64, dim3(nb), dim3(256), 0, h->stream, h->r32, h->r,
+                         h->pli32, h->pli, ns);
     if ((rc = final_residual(h))) return rc;
-    SolveState cur;
-    FC3D_HIP_TRY(hipMemcpyAsync(&cur, h->st, sizeof cur, hipMemcpyDeviceToHost, h->stream));
+    FC3D_HIP_TRY(hipMemcpyAsync(&after[trial], h->st, sizeof(SolveState), hipMemcpyDeviceToHost,
+                                h->stream));
     FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
-    res[trial] = cur.full;
+    res[trial] = after[trial].full;
     if (trial == 0) {
       FC3D_HIP_TRY(hipMemcpyAsync(h->rbak, h->r, sizeof(double4) * ns, hipMemcpyDeviceToDevice,
                                   h->stream));
@@ -1179,11 +1322,11 @@ static int learned_start(fc3d_hip_problem* h, int phase, int schedule, int max_i
   out->warm_start_used = keep ? 2 : (previous ? 0 : 1);
   out->start_residual_fallback = res[0];
   out->start_residual_learned = res[1];
-  if (!s0->stop) {
-    s0->iter = 1;
-    s0->fp32_iter = phase ? 1 : 0;
-    s0->stop = s0->iter >= max_iter ? 1 : 0;
-  }
+  out->start_trial_sweeps = after[keep ? 0 : 1].iter;  // the discarded leg's sweeps
+  *s0 = after[keep];
+  // the kept leg's state: its sweeps count; its residual field held the
+  // trial's final residual, which the convergence logic recomputes
+  s0->full = INFINITY;
   FC3D_HIP_TRY(hipMemcpyAsync(h->st, s0, sizeof *s0, hipMemcpyHostToDevice, h->stream));
   FC3D_HIP_TRY(hipGetLastError());
   return 0;
@@ -1486,10 +1629,13 @@ void fc3d_hip_problem_destroy(fc3d_hip_problem* h) {
   for (void* p : ptrs)
     if (p) hipFree(p);
   free_accel(h);
+  free_stage_bufs(h);
   void* dfp[] = {h->dep_ptr, h->dep_idx, h->qlist, h->done,    h->df,    h->d_block_dst,
                  h->d_o2s,    h->d_cps,    h->d_ba,   h->d_bb,   h->d_inc_ptr, h->d_inc, h->d_H,
                  h->d_minv,   h->d_vfree,  h->d_V,    h->d_H32,     h->d_minv32, h->d_V32,
-                 h->d_F,      h->d_F32,    h->d_frz,  h->d_le2,  h->d_fcnt};
+                 h->d_F,      h->d_F32,    h->d_frz,  h->d_le2,  h->d_fcnt,
+                 h->d_fs,     h->d_bmark,  h->d_slice_colour, h->d_focus_cnt,
+                 h->d_focus_list, h->d_fdiff32, h->d_fdiff64};
   for (void* p : dfp)
     if (p) hipFree(p);
   if (h->stream) hipStreamDestroy(h->stream);
@@ -1896,9 +2042,14 @@ int fc3d_hip_solve_observed(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_
               "(fc3d_hip_problem_set_bodies)");
     return NUMERICS_ERR_INVALID_OPTION;
   }
-  if (p->velocity_form && p->warm_start >= 2) {
-    set_error("fc3d_hip_solve: velocity_form with a learned start is not available");
-    return NUMERICS_ERR_NOT_IMPLEMENTED;
+  if (p->focus_passes < 0 || (p->focus_passes > 0 &&
+                               (!p->velocity_form || h->frames || p->accel_depth > 0 ||
+                                p->focus_every < 1 || p->focus_count < 1 || p->focus_after < 0 ||
+                                !(p->focus_share >= 0.0)))) {
+    set_error("fc3d_hip_solve: focused passes need the velocity form (general H blocks), no "
+              "acceleration, focus_every >= 1, focus_count >= 1, focus_after >= 0 and "
+              "focus_share >= 0");
+    return NUMERICS_ERR_INVALID_ARGUMENT;
   }
   if (p->freeze_contacts < 0 ||
       (p->freeze_contacts > 0 && (p->precision != 0 || p->velocity_form || p->accel_depth > 0 ||
@@ -1924,6 +2075,10 @@ int fc3d_hip_solve_observed(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_
   dp.freeze = p->freeze_contacts;
   dp.nc = h->nc;
   dp.freeze_tmp = h->freeze_tmp;
+  dp.focus_every = p->focus_every;
+  dp.focus_after = p->focus_after;
+  dp.focus_count = p->focus_count;
+  dp.focus_share = p->focus_share;
   FC3D_HIP_TRY(hipMemcpyAsync(h->prm, &dp, sizeof dp, hipMemcpyHostToDevice, h->stream));
   if (p->warm_start == 1)
     FC3D_HIP_TRY(hipMemsetAsync(h->r, 0, sizeof(double4) * (size_t)h->nslots, h->stream));
@@ -1967,6 +2122,12 @@ int fc3d_hip_solve_observed(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_
   if (p->accel_depth > 0) {
     if ((rc = ensure_accel(h, mixed))) return rc;
     h->accel_on = 1;
+  }
+  h->focus_passes = 0;
+  if (p->velocity_form && p->focus_passes > 0) {
+    if ((rc = ensure_focus(h, mixed))) return rc;
+    hipLaunchKernelGGL(k_focus_reset, dim3(1), dim3(64), 0, h->stream, h->d_fs);
+    h->focus_passes = p->focus_passes;
   }
   if (p->velocity_form) {
     if (mixed && (rc = ensure_vel32(h))) return rc;
@@ -2012,7 +2173,8 @@ int fc3d_hip_solve_observed(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_
   }
   FC3D_HIP_TRY(hipEventRecord(t0, h->stream));
   if (p->warm_start >= 2 &&
-      (rc = learned_start(h, mixed, sched[mixed], p->max_iter, p->warm_start == 3, &s0, s)))
+      (rc = learned_start(h, mixed, sched[mixed], p->start_trial_sweeps, p->warm_start == 3, &s0,
+                          s)))
     return rc;
   const int chunk = p->sweeps_per_launch > 0 ? p->sweeps_per_launch : 1;
   // FC3D_HIP_NO_GRAPH=1 enqueues the same kernels directly instead of
@@ -2052,6 +2214,7 @@ int fc3d_hip_solve_observed(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_
       }
       if (p->instrument) {
         enqueue_sweep(h, phase, sched[phase], kev.data(), (int)kev.size());
+        if (h->focus_passes > 0) enqueue_focus(h, phase);
         FC3D_HIP_TRY(hipGetLastError());
         FC3D_HIP_TRY(hipMemcpyAsync(&cur, h->st, sizeof cur, hipMemcpyDeviceToHost, h->stream));
         FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
@@ -2075,6 +2238,7 @@ int fc3d_hip_solve_observed(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_
         }
       } else if (no_graph) {
         for (int k = 0; k < chunk; k++) enqueue_sweep(h, phase, sched[phase], nullptr, 0);
+        if (h->focus_passes > 0) enqueue_focus(h, phase);
         FC3D_HIP_TRY(hipGetLastError());
         FC3D_HIP_TRY(hipMemcpyAsync(&cur, h->st, sizeof cur, hipMemcpyDeviceToHost, h->stream));
         FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
@@ -2120,6 +2284,14 @@ int fc3d_hip_solve_observed(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_
   hipEventDestroy(t0);
   hipEventDestroy(t1);
   for (auto& e : kev) hipEventDestroy(e);
+  if (h->focus_passes > 0) {
+    FocusState fst;
+    FC3D_HIP_TRY(hipMemcpy(&fst, h->d_fs, sizeof fst, hipMemcpyDeviceToHost));
+    s->focus_rounds = fst.rounds;
+    s->focus_listed = fst.listed_sum;
+    s->focus_due = fst.due;
+    s->focus_last_share = fst.last_share;
+  }
   s->solve_ms = ms;
   s->iterations = fin.iter;
   s->incr_error = incr;

@@ -70,6 +70,12 @@ void fc3d_hip_params_default(fc3d_hip_params* p) {
   p->velocity_form = 0;
   p->fp32_local_solver = 0;
   p->freeze_contacts = 0;
+  p->focus_passes = 0;
+  p->focus_every = 16;
+  p->focus_after = 64;
+  p->focus_count = 256;
+  p->focus_share = 0.5;
+  p->start_trial_sweeps = 1;
 }
 
 }  // extern "C"

@@ -28,6 +28,10 @@ struct DevParams {
   int filter;
   int relaxation;
   int rho_strategy;
+  int focus_every;      // focused passes: a round after every focus_every-th sweep
+  int focus_after;      // ... from this sweep on
+  int focus_count;      // contacts of largest |r_new - r_old|^2 that seed a round
+  double focus_share;   // least share of sum |r_new - r_old|^2 they must carry
   int freeze;           // contact freezing: sweeps a frozen contact is skipped (0: off)
   int nc;               // contacts of the problem (freezing's reset rule)
   double freeze_tmp;    // nc * nc / 1000 (fc3d_nsgs.c:722)
@@ -50,7 +54,7 @@ struct SolveState {
   double nf;      // sqrt of the last sweep's sum |r_new - r_old|^2
   double norm_r;  // |r| after the last light-error evaluation (1e24 before)
   int unfreeze;   // contact freezing: every counter reads 0 in this sweep
-  int pad2;
+  int focus_last; // focused passes: sweep count at the last round
 };
 
 template <typename T>

@@ -34,6 +34,8 @@ struct BodyPtrs {
   const double* vfree;        // per body: 6
   const int* inc_ptr;         // per body: range in inc
   const int* inc;             // 2 slot + side, ascending slot
+  // focused passes: r_new - r_old of the sweep that ends a focus interval
+  typename vec4<T>::type* fdiff = nullptr;
 };
 
 template <typename T>
@@ -164,22 +166,25 @@ __global__ void k_vinit(BodyPtrs<T> B, const typename vec4<T>::type* __restrict_
   B.V[2 * (size_t)k + 1] = v1;
 }
 
-// One slice (64 contacts) of one colour in the velocity form, for the
-// calling wave; writes the slice's partial sums.  With CHECK, the stop flag
-// is tested after the slot's loads are issued (the caller tests it
-// otherwise) and the function returns without writing if it is set.
+// The velocity-form step of one contact, slot s * 64 + lane (s need not be
+// wave-uniform): returns false, with nothing written, when CHECK finds the
+// stop flag set; light and nr2 receive the slot's light-error term and
+// squared reaction (0 for padding).  With B.fdiff set, the sweep that ends a
+// focus interval also stores r_new - r_old.
 template <typename T, bool CHECK, bool FRAMES>
-__device__ __forceinline__ void vel_slice(const SweepPtrs<T>& P, const BodyPtrs<T>& B,
-                                          const DevParams* __restrict__ prm,
-                                          const SolveState* __restrict__ st, int s, int lane,
-                                          int want_phase) {
+__device__ __forceinline__ bool vel_slot(const SweepPtrs<T>& P, const BodyPtrs<T>& B,
+                                         const DevParams* __restrict__ prm,
+                                         const SolveState* __restrict__ st, int s, int lane,
+                                         int want_phase, double& light, double& nr2) {
   const int slot = s * 64 + lane;
-  double light = 0.0, nr2 = 0.0;
+  light = 0.0;
+  nr2 = 0.0;
   // Every load of the slot is issued before the stop flag is tested, so the
   // flag's latency overlaps theirs (all addresses are in bounds: padding
   // slots read body 0).  The empty asm makes the loaded values live before
   // the test, which keeps the compiler from sinking the loads below it.
   const int stop = CHECK ? st->stop : 0, phase = CHECK ? st->phase : want_phase;
+  const bool record = B.fdiff && st->iter + 1 - st->focus_last >= prm->focus_every;
   const StepConsts sc = step_consts<T>(prm);
   const T tol = T(want_phase ? prm->fp32_local_tol : st->local_tol);
   const int len = P.len[slot];
@@ -212,7 +217,7 @@ __device__ __forceinline__ void vel_slice(const SweepPtrs<T>& P, const BodyPtrs<
     if (FRAMES) asm volatile("" ::"v"(mk[0][0]), "v"(mk[1][5]));
     asm volatile("" ::"v"(len), "v"(in.lp.mu), "v"(in.ro.x), "v"(in.pli), "v"(h[0]), "v"(h[35]),
                  "v"(va[0][0].x), "v"(va[1][1].x), "s"(stop), "s"(phase));
-    if (stop || phase != want_phase) return;
+    if (stop || phase != want_phase) return false;
   }
   if (len >= 0) {
     if (FRAMES) wcc_from_h<T>(B, kb, mk, h, in.lp.W);
@@ -236,6 +241,11 @@ __device__ __forceinline__ void vel_slice(const SweepPtrs<T>& P, const BodyPtrs<
     typename vec4<T>::type out;
     finish_slot<T>(P, sc, s, lane, tol, in, &light, &nr2, &out);
     const T dr[3] = {out.x - rold[0], out.y - rold[1], out.z - rold[2]};
+    if (record) {
+      typename vec4<T>::type d;
+      d.x = dr[0]; d.y = dr[1]; d.z = dr[2]; d.w = T(0);
+      B.fdiff[slot] = d;
+    }
     // dv = M^-1 H^T dr for both sides.  With diagonal M^-1 both sides'
     // entries are loaded at once, so the update waits on one round trip.
     // FP32 keeps the body velocities loaded before the solve; FP64 reads
@@ -304,6 +314,20 @@ __device__ __forceinline__ void vel_slice(const SweepPtrs<T>& P, const BodyPtrs<
       B.V[2 * (size_t)kb[side] + 1] = v1;
     }
   }
+  return true;
+}
+
+// One slice (64 contacts) of one colour in the velocity form, for the
+// calling wave; writes the slice's partial sums.  With CHECK, the stop flag
+// is tested after the slot's loads are issued (the caller tests it
+// otherwise) and the function returns without writing if it is set.
+template <typename T, bool CHECK, bool FRAMES>
+__device__ __forceinline__ void vel_slice(const SweepPtrs<T>& P, const BodyPtrs<T>& B,
+                                          const DevParams* __restrict__ prm,
+                                          const SolveState* __restrict__ st, int s, int lane,
+                                          int want_phase) {
+  double light, nr2;
+  if (!vel_slot<T, CHECK, FRAMES>(P, B, prm, st, s, lane, want_phase, light, nr2)) return;
   light = wave_sum(light);
   nr2 = wave_sum(nr2);
   if (lane == 0) {

@@ -92,6 +92,8 @@ int fc3d_nsgs_hip_params_from_options(const SolverOptions* o, fc3d_hip_params* p
     p->fp32_local_tol = o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_FP32_LOCAL_TOL];
   p->freeze_contacts =
       o->iparam[SICONOS_NSGS_FREEZING_CONTACT] > 0 ? o->iparam[SICONOS_NSGS_FREEZING_CONTACT] : 0;
+  if (o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_START_TRIAL_SWEEPS] >= 1.0)
+    p->start_trial_sweeps = (int)o->dparam[SICONOS_FRICTION_3D_NSGS_HIP_START_TRIAL_SWEEPS];
   return 0;
 }
 

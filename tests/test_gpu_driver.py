@@ -231,7 +231,7 @@ def test_driver_keeps_the_device_problem_between_calls(oracle):
     np.testing.assert_array_equal(r4, H.reaction())
     # residency 1: an unchanged non-zero version keeps the device values
     o.dparam[28] = 1
-    S.version.number = 7
+    S.version = 7
     assert N.fc3d_driver(p, r4, u4, o) == 0  # uploads (version seen first)
     _, _, va = _stats()
     vals = g.W.values.reshape(-1, 9)
@@ -243,7 +243,7 @@ def test_driver_keeps_the_device_problem_between_calls(oracle):
     _, _, vb = _stats()
     assert vb == va  # version 7 unchanged: values not re-uploaded
     np.testing.assert_array_equal(r5, H.reaction())
-    S.version.number = 8
+    S.version = 8
     assert N.fc3d_driver(p, r5, u5, o) == 0
     np.testing.assert_array_equal(r5, r3)
     capi.load().fc3d_hip_resident_clear(None)

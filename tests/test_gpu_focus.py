@@ -1,0 +1,56 @@
+"""Focused passes of the velocity form (fc3d_hip_params.focus_*): extra
+Gauss-Seidel passes over the few contact clusters that still move, on a pile
+whose plain sweeps spend most of their time on them.  Judged by the oracle's
+fc3d_compute_error; the selection uses atomics, so the results must not
+depend on their order (two solves give the same bits)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from siconos_amd import capi
+from siconos_amd import numerics as N
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pile():
+    # 50k contacts whose plain sweeps need ~2400 sweeps, most of them for a
+    # few clusters of 5-7 bodies
+    g = N.generate_granular_pile(20_000, 50_000, 0.1, 0.9, 101)
+    H = N.HipProblem(g.W, g.q, g.mu)
+    H.set_bodies(g.nbodies, g.body_a, g.body_b, g.H, g.minv, g.vfree)
+    return g, H
+
+
+FOCUS = dict(focus_passes=50, focus_every=16, focus_after=64, focus_count=128, focus_share=0.5)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_focused_passes_converge_in_fewer_sweeps(pile, oracle, precision):
+    g, H = pile
+    base = dict(tol=1e-6, max_iter=6000, warm_start=1, velocity_form=1, precision=precision,
+                fp32_tol=1e-6, fp32_local_solver=550 if precision else 0, fp32_local_tol=1e-7)
+    s0 = H.solve(N.hip_params(**base))
+    assert s0.info == 0
+    s1 = H.solve(N.hip_params(**base, **FOCUS))
+    r1 = H.reaction()
+    print(f"precision {precision}: {s0.iterations} sweeps plain, {s1.iterations} focused "
+          f"({s1.focus_rounds} rounds of {s1.focus_due} due, {s1.focus_listed} contacts listed, "
+          f"last share {s1.focus_last_share:.3f})", flush=True)
+    assert s1.info == 0 and s1.residual <= 1e-6
+    assert s1.focus_rounds > 0
+    assert s1.iterations < 0.5 * s0.iterations
+    e, _ = oracle.compute_error(g.W, g.q, g.mu, r1)
+    assert e <= 1e-6 and e == pytest.approx(s1.residual, rel=1e-9)
+    s2 = H.solve(N.hip_params(**base, **FOCUS))
+    assert s2.iterations == s1.iterations and s2.focus_listed == s1.focus_listed
+    np.testing.assert_array_equal(H.reaction(), r1)
+
+
+def test_focused_passes_need_the_velocity_form(pile):
+    g, H = pile
+    rc = capi.load().fc3d_hip_solve(H._h, C.byref(N.hip_params(**FOCUS)),
+                                    C.byref(capi.HipStats()))
+    assert rc == -2
