@@ -1932,19 +1932,21 @@ int fc3d_hip_problem_set_bodies(fc3d_hip_problem* h, int nbodies, const int* bod
   h->frames = frames ? 1 : 0;
   h->nbodies = nbodies;
   h->minv_diag = minv_diagonal ? 1 : 0;
-  // algorithmic bytes per contact of one velocity-form colour pass, in
-  // values of the sweep precision: H (36) and W_cc (9), or in the frames
-  // layout T and the lever arms (15) with W_cc recomputed; mu (1), r read
-  // and written (4 + 4, padded vec4), PLI step read and written (2); per
-  // body side present: its velocity read and written (8 + 8, two padded
-  // vec4) and M^-1 (6 or 36); plus len and the two body ids (12 bytes)
+  // algorithmic bytes per contact of one velocity-form colour pass, each
+  // unique value counted once (the padding of the 4-wide r and v vectors is
+  // not), in values of the sweep precision: H (36) and W_cc (9), or in the
+  // frames layout T and the lever arms (15) with W_cc recomputed; mu (1), r
+  // read and written (3 + 3), PLI step read and written (2); per body side
+  // present: its velocity read and written (6 + 6) and M^-1 (6 or 36); plus
+  // len and the two body ids (12 bytes).  A two-body FP32 contact without the
+  // PLI state moves 364 bytes.
   const double mb = minv_diagonal ? 6.0 : 36.0;
   h->vel_bytes64.assign((size_t)h->ncolours, 0.0);
   h->vel_bytes32.assign((size_t)h->ncolours, 0.0);
   for (int c = 0; c < nc; c++) {
     const int col = h->colour[(size_t)c];
     const double sides = (body_a[c] >= 0 ? 1.0 : 0.0) + (body_b[c] >= 0 ? 1.0 : 0.0);
-    const double vals = (frames ? 15 : 36 + 9) + 1 + 8 + 2 + sides * (16 + mb);
+    const double vals = (frames ? 15 : 36 + 9) + 1 + 6 + 2 + sides * (12 + mb);
     h->vel_bytes64[(size_t)col] += 8.0 * vals + 12.0;
     h->vel_bytes32[(size_t)col] += 4.0 * vals + 12.0;
   }
