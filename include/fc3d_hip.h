@@ -208,6 +208,26 @@ FC3D_API int fc3d_hip_solve_observed(fc3d_hip_problem* h, const fc3d_hip_params*
 /* fc3d_compute_error.c:70-122 on the resident reaction; writes the resident
  * velocity.  norm < 0 uses the stored |q|. */
 FC3D_API int fc3d_hip_compute_error(fc3d_hip_problem* h, double norm, double* error);
+/* fc3d_compute_error on device arrays, allocating nothing.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Every pointer is device memory the caller owns: W as CSR over 3x3 blocks
+ * (row_ptr nc + 1 entries, col_idx and blocks nnzb, column-major blocks),
+ * q [3nc], mu [nc], reaction [3nc] in; velocity [3nc] (u = q + W r, each
+ * row's blocks added in their stored order, as fc3d_compute_error does with
+ * SBM_gemv_3x3) and error [1] (fc3d_compute_error.c:70-122 relative to
+ * max(norm, |r|, |u|)) out.  The kernels are queued on `stream` (a
+ * hipStream_t; NULL: the default stream) and the call returns without
+ * waiting.  nc is limited to 4,194,304 contacts (larger is refused): the
+ * partial sums use a fixed 1.5 MiB device array of the library on each GPU,
+ * so calls that overlap on one GPU race.  A row whose range or column indices
+ * fall outside [0, nnzb) or [0, nc) yields a NaN velocity and error. */
+FC3D_API int fc3d_hip_compute_error_device(int nc, int64_t nnzb, const int64_t* row_ptr,
+                                           const int64_t* col_idx, const double* blocks,
+                                           const double* q, const double* mu,
+                                           const double* reaction, double norm,
+                                           double* velocity, double* error, int device,
+                                           void* stream);
 /* SparseBlockMatrix.c:360-419: y += W x (host vectors, original order). */
 FC3D_API int fc3d_hip_gemv(fc3d_hip_problem* h, const double* x, double* y);
 

@@ -244,6 +244,10 @@ SIGNATURES = {
     "fc3d_hip_solve_observed": (C.c_int, [H_p, C.POINTER(HipParams), C.POINTER(HipStats),
                                           OBSERVER_FN, C.c_void_p]),
     "fc3d_hip_compute_error": (C.c_int, [H_p, C.c_double, c_double_p]),
+    "fc3d_hip_compute_error_device": (C.c_int, [C.c_int, C.c_int64, C.c_void_p, C.c_void_p,
+                                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                C.c_double, C.c_void_p, C.c_void_p, C.c_int,
+                                                C.c_void_p]),
     "fc3d_hip_resident_clear": (None, [NM_p]),
     "fc3d_hip_resident_stats": (None, [c_int64_p, c_int64_p, c_int64_p]),
     "fc3d_hip_gemv": (C.c_int, [H_p, c_double_p, c_double_p]),
