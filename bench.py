@@ -290,7 +290,8 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n, "copies": 5}))
 
 
-FOCUS = dict(focus_passes=50, focus_every=16, focus_after=64, focus_count=256, focus_share=0.5)
+FOCUS = dict(focus_passes=50, focus_every=16, focus_after=64, focus_count=256, focus_share=0.5,
+             focus_omega=1.2)
 
 
 def main():

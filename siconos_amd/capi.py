@@ -96,7 +96,8 @@ class HipParams(C.Structure):
                 ("velocity_form", C.c_int), ("fp32_local_solver", C.c_int),
                 ("freeze_contacts", C.c_int), ("focus_passes", C.c_int),
                 ("focus_every", C.c_int), ("focus_after", C.c_int), ("focus_count", C.c_int),
-                ("focus_share", C.c_double), ("start_trial_sweeps", C.c_int)]
+                ("focus_share", C.c_double), ("start_trial_sweeps", C.c_int),
+                ("focus_omega", C.c_double)]
 
 
 class HipStats(C.Structure):

@@ -19,7 +19,8 @@
 //      k_focus_collect lists, per colour, every contact touching a marked
 //      body (the clusters and their first ring);
 //   4. k_focus_passes runs `passes` Gauss-Seidel passes over the list in one
-//      workgroup, colour after colour, with the sweep's own per-contact step.
+//      workgroup, colour after colour, with the sweep's own per-contact step
+//      and the passes' own relaxation (DevParams::focus_omega).
 // k_focus_clear, k_focus_hist and k_focus_threshold test whether a round is
 // due; the threshold kernel records the round in SolveState::focus_last and
 // sets FocusState::active, which the later kernels follow.
@@ -380,7 +381,7 @@ __global__ __launch_bounds__(vel_tail_threads<T>()) void k_focus_passes_staged(
   if (threadIdx.x == 0) focus_offsets(cnt, ncolours, capc, off);
   __syncthreads();
   if (off[0] < 0 || off[ncolours] == 0) return;
-  const StepConsts sc = step_consts<T>(prm);
+  const StepConsts sc = focus_consts<T>(prm);
   const T tol = T(want_phase ? prm->fp32_local_tol : st->local_tol);
   for (int pass = 0; pass < passes; pass++) {
     for (int c = 0; c < ncolours; c++) {
@@ -420,7 +421,8 @@ __global__ __launch_bounds__(vel_tail_threads<T>()) void k_focus_passes(
       for (int i = threadIdx.x; i < n; i += nt) {
         const int slot = lc[i];
         double light, nr2;
-        vel_slot<T, false, false>(P, B, prm, st, slot >> 6, slot & 63, want_phase, light, nr2);
+        vel_slot<T, false, false, true>(P, B, prm, st, slot >> 6, slot & 63, want_phase, light,
+                                        nr2);
       }
       __syncthreads();
     }

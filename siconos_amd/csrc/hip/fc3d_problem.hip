@@ -2047,10 +2047,11 @@ int fc3d_hip_solve_observed(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_
   if (p->focus_passes < 0 || (p->focus_passes > 0 &&
                                (!p->velocity_form || h->frames || p->accel_depth > 0 ||
                                 p->focus_every < 1 || p->focus_count < 1 || p->focus_after < 0 ||
-                                !(p->focus_share >= 0.0)))) {
+                                !(p->focus_share >= 0.0) || !(p->focus_omega > 0.0) ||
+                                !(p->focus_omega < 2.0)))) {
     set_error("fc3d_hip_solve: focused passes need the velocity form (general H blocks), no "
-              "acceleration, focus_every >= 1, focus_count >= 1, focus_after >= 0 and "
-              "focus_share >= 0");
+              "acceleration, focus_every >= 1, focus_count >= 1, focus_after >= 0, "
+              "focus_share >= 0 and 0 < focus_omega < 2");
     return NUMERICS_ERR_INVALID_ARGUMENT;
   }
   if (p->freeze_contacts < 0 ||
@@ -2081,6 +2082,7 @@ int fc3d_hip_solve_observed(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_
   dp.focus_after = p->focus_after;
   dp.focus_count = p->focus_count;
   dp.focus_share = p->focus_share;
+  dp.focus_omega = p->focus_omega;
   FC3D_HIP_TRY(hipMemcpyAsync(h->prm, &dp, sizeof dp, hipMemcpyHostToDevice, h->stream));
   if (p->warm_start == 1)
     FC3D_HIP_TRY(hipMemsetAsync(h->r, 0, sizeof(double4) * (size_t)h->nslots, h->stream));

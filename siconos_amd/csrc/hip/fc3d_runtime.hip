@@ -76,6 +76,7 @@ void fc3d_hip_params_default(fc3d_hip_params* p) {
   p->focus_count = 256;
   p->focus_share = 0.5;
   p->start_trial_sweeps = 1;
+  p->focus_omega = 1.2;
 }
 
 }  // extern "C"

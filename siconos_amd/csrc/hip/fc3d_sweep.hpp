@@ -32,6 +32,7 @@ struct DevParams {
   int focus_after;      // ... from this sweep on
   int focus_count;      // contacts of largest |r_new - r_old|^2 that seed a round
   double focus_share;   // least share of sum |r_new - r_old|^2 they must carry
+  double focus_omega;   // relaxation of the focus passes (1: none)
   int freeze;           // contact freezing: sweeps a frozen contact is skipped (0: off)
   int nc;               // contacts of the problem (freezing's reset rule)
   double freeze_tmp;    // nc * nc / 1000 (fc3d_nsgs.c:722)
@@ -375,6 +376,18 @@ __device__ __forceinline__ StepConsts step_consts(const DevParams* __restrict__ 
   c.filter = prm->filter;
   c.rho_constant = prm->rho_constant;
   c.omega = prm->omega;
+  return c;
+}
+
+// The step constants of the focused passes: the sweep's, with the passes' own
+// relaxation when focus_omega is not 1.
+template <typename T>
+__device__ __forceinline__ StepConsts focus_consts(const DevParams* __restrict__ prm) {
+  StepConsts c = step_consts<T>(prm);
+  if (prm->focus_omega != 1.0) {
+    c.relaxation = 1;
+    c.omega = prm->focus_omega;
+  }
   return c;
 }
 

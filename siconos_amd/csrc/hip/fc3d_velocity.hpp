@@ -170,8 +170,9 @@ __global__ void k_vinit(BodyPtrs<T> B, const typename vec4<T>::type* __restrict_
 // wave-uniform): returns false, with nothing written, when CHECK finds the
 // stop flag set; light and nr2 receive the slot's light-error term and
 // squared reaction (0 for padding).  With B.fdiff set, the sweep that ends a
-// focus interval also stores r_new - r_old.
-template <typename T, bool CHECK, bool FRAMES>
+// focus interval also stores r_new - r_old.  FOCUS: a focused pass's step
+// (focus_consts).
+template <typename T, bool CHECK, bool FRAMES, bool FOCUS = false>
 __device__ __forceinline__ bool vel_slot(const SweepPtrs<T>& P, const BodyPtrs<T>& B,
                                          const DevParams* __restrict__ prm,
                                          const SolveState* __restrict__ st, int s, int lane,
@@ -185,7 +186,7 @@ __device__ __forceinline__ bool vel_slot(const SweepPtrs<T>& P, const BodyPtrs<T
   // the test, which keeps the compiler from sinking the loads below it.
   const int stop = CHECK ? st->stop : 0, phase = CHECK ? st->phase : want_phase;
   const bool record = B.fdiff && st->iter + 1 - st->focus_last >= prm->focus_every;
-  const StepConsts sc = step_consts<T>(prm);
+  const StepConsts sc = FOCUS ? focus_consts<T>(prm) : step_consts<T>(prm);
   const T tol = T(want_phase ? prm->fp32_local_tol : st->local_tol);
   const int len = P.len[slot];
   SlotInputs<T> in;

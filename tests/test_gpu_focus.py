@@ -49,6 +49,33 @@ def test_focused_passes_converge_in_fewer_sweeps(pile, oracle, precision):
     np.testing.assert_array_equal(H.reaction(), r1)
 
 
+@pytest.mark.parametrize("omega", [1.0, 1.3])
+def test_focus_relaxation(pile, oracle, omega):
+    """focus_omega relaxes the focus passes' steps only; without it (1.0) or
+    with more of it the solve still converges, and gives the same bits twice."""
+    g, H = pile
+    base = dict(tol=1e-6, max_iter=6000, warm_start=1, velocity_form=1, precision=1,
+                fp32_tol=1e-6, fp32_local_solver=550, fp32_local_tol=1e-7)
+    s1 = H.solve(N.hip_params(**base, **FOCUS, focus_omega=omega))
+    r1 = H.reaction()
+    print(f"focus_omega {omega}: {s1.iterations} sweeps, {s1.focus_rounds} rounds", flush=True)
+    assert s1.info == 0 and s1.residual <= 1e-6 and s1.focus_rounds > 0
+    e, _ = oracle.compute_error(g.W, g.q, g.mu, r1)
+    assert e <= 1e-6 and e == pytest.approx(s1.residual, rel=1e-9)
+    s2 = H.solve(N.hip_params(**base, **FOCUS, focus_omega=omega))
+    assert s2.iterations == s1.iterations
+    np.testing.assert_array_equal(H.reaction(), r1)
+
+
+@pytest.mark.parametrize("omega", [0.0, 2.0, float("nan")])
+def test_focus_omega_outside_0_2_is_refused(pile, omega):
+    g, H = pile
+    p = N.hip_params(velocity_form=1, **FOCUS, focus_omega=omega)
+    rc = capi.load().fc3d_hip_solve(H._h, C.byref(p), C.byref(capi.HipStats()))
+    assert rc == -2
+    assert b"focus_omega" in capi.load().fc3d_hip_last_error()
+
+
 def test_focused_passes_need_the_velocity_form(pile):
     g, H = pile
     rc = capi.load().fc3d_hip_solve(H._h, C.byref(N.hip_params(**FOCUS)),
