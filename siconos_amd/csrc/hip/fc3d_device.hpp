@@ -16,6 +16,23 @@ template <typename T> struct num;
 template <> struct num<double> { static constexpr double eps = DBL_EPSILON; };
 template <> struct num<float> { static constexpr float eps = FLT_EPSILON; };
 
+// Quotients and square roots of the local solvers: IEEE in double (the
+// reference's results bit for bit); in float the hardware's reciprocal and
+// square root (v_rcp_f32, v_sqrt_f32, about 1 ulp), two instructions and one
+// in place of the correctly rounded sequences of about ten, which sit on the
+// dependent chain of every FP32 contact step.  The FP32 sweeps are refined
+// and judged in FP64, so their last bit is not part of any result.
+template <typename T>
+__device__ __forceinline__ T qdiv(T a, T b) { return a / b; }
+template <>
+__device__ __forceinline__ float qdiv<float>(float a, float b) {
+  return a * __builtin_amdgcn_rcpf(b);
+}
+template <typename T>
+__device__ __forceinline__ T qsqrt(T x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float qsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // op3x3.h:342-363 — r += a v, column by column
 template <typename T>
 __device__ __forceinline__ void mvp3x3(const T* a, const T* v, T* r) {
@@ -49,15 +66,15 @@ __device__ __forceinline__ int solve_3x3_gepp(const G* a, G* b, G* minpiv = null
   const G aa0 = fabs(a0), aa1 = fabs(a1), aa2 = fabs(a2);
   const int p1 = aa0 >= aa1 ? (aa0 >= aa2 ? 0 : 2) : (aa1 >= aa2 ? 1 : 2);
   if (p1 == 0) {
-    f1 = a1 / a0; f2 = a2 / a0; lp0 = a0; alp0 = fabs(a0); lp1 = a[3]; lp2 = a[6];
+    f1 = qdiv<G>(a1, a0); f2 = qdiv<G>(a2, a0); lp0 = a0; alp0 = fabs(a0); lp1 = a[3]; lp2 = a[6];
     lm1 = a[4] - f1 * lp1; lm2 = a[7] - f1 * lp2; ln1 = a[5] - f2 * lp1; ln2 = a[8] - f2 * lp2;
     bl = b[0]; bm = b[1] - f1 * bl; bn = b[2] - f2 * bl;
   } else if (p1 == 1) {
-    f1 = a0 / a1; f2 = a2 / a1; lp0 = a1; alp0 = fabs(a1); lp1 = a[4]; lp2 = a[7];
+    f1 = qdiv<G>(a0, a1); f2 = qdiv<G>(a2, a1); lp0 = a1; alp0 = fabs(a1); lp1 = a[4]; lp2 = a[7];
     lm1 = a[3] - f1 * lp1; lm2 = a[6] - f1 * lp2; ln1 = a[5] - f2 * lp1; ln2 = a[8] - f2 * lp2;
     bl = b[1]; bm = b[0] - f1 * bl; bn = b[2] - f2 * bl;
   } else {
-    f1 = a0 / a2; f2 = a1 / a2; lp0 = a2; alp0 = fabs(a2); lp1 = a[5]; lp2 = a[8];
+    f1 = qdiv<G>(a0, a2); f2 = qdiv<G>(a1, a2); lp0 = a2; alp0 = fabs(a2); lp1 = a[5]; lp2 = a[8];
     lm1 = a[3] - f1 * lp1; lm2 = a[6] - f1 * lp2; ln1 = a[4] - f2 * lp1; ln2 = a[7] - f2 * lp2;
     bl = b[2]; bm = b[0] - f1 * bl; bn = b[1] - f2 * bl;
   }
@@ -66,19 +83,19 @@ __device__ __forceinline__ int solve_3x3_gepp(const G* a, G* b, G* minpiv = null
   G p2;
   if (alm1 >= aln1) {
     if (alm1 < G(DBL_EPSILON)) return 1;
-    f1 = ln1 / lm1;
+    f1 = qdiv<G>(ln1, lm1);
     p2 = ln2 - f1 * lm2;
-    s2 = (bn - f1 * bm) / p2;
-    s1 = (bm - lm2 * s2) / lm1;
+    s2 = qdiv<G>(bn - f1 * bm, p2);
+    s1 = qdiv<G>(bm - lm2 * s2, lm1);
   } else {
     if (aln1 < G(DBL_EPSILON)) return 1;
-    f1 = lm1 / ln1;
+    f1 = qdiv<G>(lm1, ln1);
     p2 = lm2 - f1 * ln2;
-    s2 = (bm - f1 * bn) / p2;
-    s1 = (bn - ln2 * s2) / ln1;
+    s2 = qdiv<G>(bm - f1 * bn, p2);
+    s1 = qdiv<G>(bn - ln2 * s2, ln1);
   }
   if (minpiv) *minpiv = fmin(fmin(alp0, fmax(alm1, aln1)), fabs(p2));
-  s0 = (bl - s1 * lp1 - s2 * lp2) / lp0;
+  s0 = qdiv<G>(bl - s1 * lp1 - s2 * lp2, lp0);
   b[0] = s0; b[1] = s1; b[2] = s2;
   return 0;
 }
@@ -86,7 +103,7 @@ __device__ __forceinline__ int solve_3x3_gepp(const G* a, G* b, G* minpiv = null
 // projectionOnCone.c:32-50
 template <typename T>
 __device__ __forceinline__ unsigned projection_on_cone(T* r, T mu) {
-  const T nt = sqrt(r[1] * r[1] + r[2] * r[2]);
+  const T nt = qsqrt<T>(r[1] * r[1] + r[2] * r[2]);
   if (mu * nt <= -r[0]) {
     r[0] = T(0); r[1] = T(0); r[2] = T(0);
     return 0u;
@@ -94,9 +111,9 @@ __device__ __forceinline__ unsigned projection_on_cone(T* r, T mu) {
     return 1u;
   } else {
     const T mu2 = mu * mu;
-    r[0] = (mu * nt + r[0]) / (mu2 + T(1));
-    r[1] = mu * r[0] * r[1] / nt;
-    r[2] = mu * r[0] * r[2] / nt;
+    r[0] = qdiv<T>(mu * nt + r[0], mu2 + T(1));
+    r[1] = qdiv<T>(mu * r[0] * r[1], nt);
+    r[2] = qdiv<T>(mu * r[0] * r[2], nt);
     return 2u;
   }
 }
@@ -109,7 +126,7 @@ __device__ __forceinline__ void ac_std(const T* R, const T* u, T mu, const T* rh
   const T RVN = R[0] - RhoN * u[0];
   const T RVT = R[1] - RhoT * u[1];
   const T RVS = R[2] - RhoT * u[2];
-  const T RV = sqrt(RVT * RVT + RVS * RVS);
+  const T RV = qsqrt<T>(RVT * RVT + RVS * RVS);
   T Radius;
   if (AB) { A[3] = T(0); A[6] = T(0); A[1] = T(0); A[2] = T(0);
             B[3] = T(0); B[6] = T(0); B[1] = T(0); B[2] = T(0); }
@@ -131,7 +148,7 @@ __device__ __forceinline__ void ac_std(const T* R, const T* u, T mu, const T* rh
     }
   } else if (RV > Radius) {
     if (Radius > T(0)) {
-      const T RV1 = T(1) / RV;
+      const T RV1 = qdiv<T>(T(1), RV);
       F[1] = R[1] - Radius * RVT * RV1;
       F[2] = R[2] - Radius * RVS * RV1;
       if (AB) {
@@ -172,7 +189,7 @@ __device__ __forceinline__ void ac_jeanmoreau(const T* R, const T* u, T mu, cons
   const T RVN = R[0] - RhoN * u[0];
   const T RVT = R[1] - RhoT * u[1];
   const T RVS = R[2] - RhoT * u[2];
-  const T RV = sqrt(RVT * RVT + RVS * RVS);
+  const T RV = qsqrt<T>(RVT * RVT + RVS * RVS);
   const T Radius = mu * R[0];
   if (RVN >= T(0)) {
     F[0] = RhoN * u[0];
@@ -189,7 +206,7 @@ __device__ __forceinline__ void ac_jeanmoreau(const T* R, const T* u, T mu, cons
       B[1] = T(0); B[4] = T(0); B[7] = T(0); B[2] = T(0); B[5] = T(0); B[8] = T(0);
     }
   } else {
-    const T RV1 = T(1) / RV;
+    const T RV1 = qdiv<T>(T(1), RV);
     F[1] = R[1] - Radius * RVT * RV1;
     F[2] = R[2] - Radius * RVS * RV1;
     if (AB) {
@@ -272,19 +289,19 @@ __device__ __forceinline__ void compute_rho(int strategy, T rho_constant, const 
   if (strategy == 1) {
     T eig[3];
     eig_3x3<T>(W, eig);
-    rho[0] = T(1) / eig[0];
+    rho[0] = qdiv<T>(T(1), eig[0]);
     rho[1] = rho[0];
     rho[2] = rho[0];
     return;
   }
   const T sw = W[1 + 1 * 3] + W[2 + 2 * 3];
   T dw = sw * sw - T(4) * (sw - W[2 + 1 * 3] + W[1 + 2 * 3]);
-  dw = dw > T(0) ? sqrt(dw) : T(0);
-  rho[0] = T(1) / W[0];
+  dw = dw > T(0) ? qsqrt<T>(dw) : T(0);
+  rho[0] = qdiv<T>(T(1), W[0]);
   if (strategy == 2)
-    rho[1] = T(2) * (sw - dw) / ((sw + dw) * (sw + dw));
+    rho[1] = qdiv<T>(T(2) * (sw - dw), (sw + dw) * (sw + dw));
   else
-    rho[1] = T(2) / (sw + dw);
+    rho[1] = qdiv<T>(T(2), sw + dw);
   rho[2] = rho[1];
 }
 
@@ -292,7 +309,7 @@ __device__ __forceinline__ void compute_rho(int strategy, T rho_constant, const 
 template <typename T>
 __device__ __forceinline__ T unitary_error(const T* r, const T* u, T mu) {
   T w[3];
-  w[0] = r[0] - u[0] - mu * sqrt(u[1] * u[1] + u[2] * u[2]);
+  w[0] = r[0] - u[0] - mu * qsqrt<T>(u[1] * u[1] + u[2] * u[2]);
   w[1] = r[1] - u[1];
   w[2] = r[2] - u[2];
   projection_on_cone<T>(w, mu);
@@ -327,11 +344,11 @@ __device__ __forceinline__ void local_velocity_inline(const LocalProblem<T>& lp,
 // onecontact...c:856-878
 template <typename T>
 __device__ __forceinline__ T compute_local_error(const LocalProblem<T>& lp, const T* r) {
-  const T nq = sqrt(lp.q[0] * lp.q[0] + lp.q[1] * lp.q[1] + lp.q[2] * lp.q[2]);
+  const T nq = qsqrt<T>(lp.q[0] * lp.q[0] + lp.q[1] * lp.q[1] + lp.q[2] * lp.q[2]);
   T v[3];
   local_velocity<T>(lp, r, v);
-  T e = sqrt(unitary_error<T>(r, v, lp.mu));
-  if (fabs(nq) > num<T>::eps) e /= nq;
+  T e = qsqrt<T>(unitary_error<T>(r, v, lp.mu));
+  if (fabs(nq) > num<T>::eps) e = qdiv<T>(e, nq);
   return e;
 }
 
@@ -381,7 +398,7 @@ __device__ void line_search_gp(const LocalProblem<T>& lp, int formulation, T* t_
     local_velocity_inline<T>(lp, tmp, v);
     ac_function<T, false>(formulation, tmp, v, lp.mu, rho, F, nullptr, nullptr);
     const T q = T(0.5) * (F[0] * F[0] + F[1] * F[1] + F[2] * F[2]);
-    const T slope = (q - q0) / alpha;
+    const T slope = qdiv<T>(q - q0, alpha);
     const bool C1 = slope >= m2 * dqdt0;
     const bool C2 = slope <= m1 * dqdt0;
     if (C1 && C2) {
@@ -404,9 +421,9 @@ __device__ void line_search_gp(const LocalProblem<T>& lp, int formulation, T* t_
 template <typename T>
 __device__ int nsn_newton(const LocalProblem<T>& lp, T* r, const LocalParams& P, T tol,
                           const T* rho, bool damped, T* residual) {
-  const T nq = sqrt(lp.q[0] * lp.q[0] + lp.q[1] * lp.q[1] + lp.q[2] * lp.q[2]);
+  const T nq = qsqrt<T>(lp.q[0] * lp.q[0] + lp.q[1] * lp.q[1] + lp.q[2] * lp.q[2]);
   T nrel = T(1);
-  if (nq > num<T>::eps) nrel /= nq;
+  if (nq > num<T>::eps) nrel = qdiv<T>(nrel, nq);
   // A and B start at zero: the JeanMoreau function leaves the coupling
   // entries untouched (fc3d_AlartCurnier_functions.c:649-753)
   T dR[3] = {T(0), T(0), T(0)}, F[3], v[3];
@@ -415,7 +432,7 @@ __device__ int nsn_newton(const LocalProblem<T>& lp, T* r, const LocalParams& P,
   local_velocity<T>(lp, r, v);
   T t = T(1);
   ac_function<T, false>(P.formulation, r, v, lp.mu, rho, F, nullptr, nullptr);
-  *residual = sqrt(F[0] * F[0] + F[1] * F[1] + F[2] * F[2]) * nrel;
+  *residual = qsqrt<T>(F[0] * F[0] + F[1] * F[1] + F[2] * F[2]) * nrel;
   if (*residual < tol) return 0;
   for (int inew = 1; inew < P.newton_max_iter; ++inew) {
     ac_function<T, true>(P.formulation, r, v, lp.mu, rho, F, A, B);
@@ -469,8 +486,8 @@ __device__ int nsn_newton(const LocalProblem<T>& lp, T* r, const LocalParams& P,
     }
     local_velocity<T>(lp, r, v);
     ac_function<T, false>(P.formulation, r, v, lp.mu, rho, F, nullptr, nullptr);
-    *residual = sqrt(F[0] * F[0] + F[1] * F[1] + F[2] * F[2]) * nrel;
-    const T ndR = sqrt(dR[0] * dR[0] + dR[1] * dR[1] + dR[2] * dR[2]);
+    *residual = qsqrt<T>(F[0] * F[0] + F[1] * F[1] + F[2] * F[2]) * nrel;
+    const T ndR = qsqrt<T>(dR[0] * dR[0] + dR[1] * dR[1] + dR[2] * dR[2]);
     if (damped) {
       if (info3) break;
       if (*residual < tol) return 0;
@@ -501,8 +518,8 @@ __device__ int pli_solve(const LocalProblem<T>& lp, T* r, int max_iter, T tol, T
     local_velocity_inline<T>(lp, r, vel_k);
     int ls_it = 0;
     bool success = false;
-    rho_k = rho / tau;
-    const T nUT = sqrt(vel_k[1] * vel_k[1] + vel_k[2] * vel_k[2]);
+    rho_k = qdiv<T>(rho, tau);
+    const T nUT = qsqrt<T>(vel_k[1] * vel_k[1] + vel_k[2] * vel_k[2]);
     while (!success && (ls_it < 10)) {
       rho_k = rho_k * tau;
       r[0] = r_k[0] - rho_k * (vel_k[0] + mu * nUT);
@@ -510,10 +527,10 @@ __device__ int pli_solve(const LocalProblem<T>& lp, T* r, int max_iter, T tol, T
       r[2] = r_k[2] - rho_k * vel_k[2];
       projection_on_cone<T>(r, mu);
       local_velocity_inline<T>(lp, r, vel);
-      a1 = sqrt((vel_k[0] - vel[0]) * (vel_k[0] - vel[0]) +
+      a1 = qsqrt<T>((vel_k[0] - vel[0]) * (vel_k[0] - vel[0]) +
                 (vel_k[1] - vel[1]) * (vel_k[1] - vel[1]) +
                 (vel_k[2] - vel[2]) * (vel_k[2] - vel[2]));
-      a2 = sqrt((r_k[0] - r[0]) * (r_k[0] - r[0]) + (r_k[1] - r[1]) * (r_k[1] - r[1]) +
+      a2 = qsqrt<T>((r_k[0] - r[0]) * (r_k[0] - r[0]) + (r_k[1] - r[1]) * (r_k[1] - r[1]) +
                 (r_k[2] - r[2]) * (r_k[2] - r[2]));
       success = (rho_k * a1 <= L * a2);
       ls_it++;
@@ -532,10 +549,10 @@ __device__ int pli_solve(const LocalProblem<T>& lp, T* r, int max_iter, T tol, T
 // fc3d_projection.c:564-603
 template <typename T>
 __device__ __forceinline__ int proj_solve(const LocalProblem<T>& lp, T* r) {
-  const T an = T(1) / lp.W[0];
+  const T an = qdiv<T>(T(1), lp.W[0]);
   T w[3];
   local_velocity_inline<T>(lp, r, w);
-  const T nUT = sqrt(w[1] * w[1] + w[2] * w[2]);
+  const T nUT = qsqrt<T>(w[1] * w[1] + w[2] * w[2]);
   r[0] -= an * (w[0] + lp.mu * nUT);
   r[1] -= an * w[1];
   r[2] -= an * w[2];

@@ -44,7 +44,7 @@ struct FocusState {
   int rounds;     // rounds with passes in this solve
   int listed_sum; // contacts listed over those rounds
   int due;        // rounds due in this solve
-  int pad;
+  int nlocal;     // local body ids handed out by this round's k_focus_stage
   double last_share;  // the last due round: share of sum |dr|^2 its selection carried
   double last_total;  // ... and that sum
 };
@@ -170,6 +170,7 @@ __global__ __launch_bounds__(FOCUS_SCAN) void k_focus_threshold(const DevParams*
   }
   fs->round += 1;
   fs->due += 1;
+  fs->nlocal = 0;
   fs->last_share = total > 0.0 ? top / total : 0.0;
   fs->last_total = total;
   fs->threshold = thr;
@@ -218,11 +219,19 @@ __global__ void k_focus_collect(const FocusState* fs, const int* __restrict__ le
 // per round into a compact buffer, entry i of colour c at off[c] + j, each
 // field a plane of N entries, so that a pass reads coalesced lines that stay
 // in L2; the reactions live there during the passes and go back at the end.
+// The bodies the entries touch get local ids 0 .. nlocal-1 (blist maps them
+// back, -1 for an id lost to a race), so that the passes can keep their
+// velocities in LDS.
 template <typename T>
 struct FocusStage {
   int* slot;   // [N] the entry's slot
   int* ba;     // [N]
   int* bb;     // [N]
+  int* la;     // [N] local id of body a (-1: none)
+  int* lb;     // [N]
+  int* blist;  // [capl] body of each local id (-1: unused)
+  unsigned long long* bkey;  // [nbodies] (round << 32) | local id of the body's last round
+  int capl;    // 2 N: at most two ids are handed out per entry
   T* H;        // [36][N]
   T* W;        // [9][N]
   T* mu;       // [N]
@@ -249,8 +258,25 @@ __device__ __forceinline__ void focus_offsets(const int* cnt, int ncolours, int 
 
 constexpr int FOCUS_MAX_COLOURS = 64;
 
+// The body's local id in this round (stamp = round << 32): the first entry
+// that claims the body with a compare-and-swap gives it a fresh id.  The ids
+// depend on the order of the claims, the passes' results do not (an id only
+// places the body's velocity in LDS).
+__device__ __forceinline__ int focus_local_body(unsigned long long* bkey, int body,
+                                                unsigned long long stamp, int* nlocal, int* blist,
+                                                int capl) {
+  const unsigned long long old = bkey[body];
+  if ((old >> 32) == (stamp >> 32)) return (int)(unsigned)old;
+  const int id = atomicAdd(nlocal, 1);
+  const unsigned long long prev = atomicCAS(&bkey[body], old, stamp | (unsigned)id);
+  // prev is old (claimed here) or another claim of this round
+  const bool won = prev == old;
+  if (id < capl) blist[id] = won ? body : -1;
+  return won ? id : (int)(unsigned)prev;
+}
+
 template <typename T>
-__global__ void k_focus_stage(const FocusState* fs, SweepPtrs<T> P, BodyPtrs<T> B,
+__global__ void k_focus_stage(FocusState* fs, SweepPtrs<T> P, BodyPtrs<T> B,
                               const int* __restrict__ cnt, const int* __restrict__ list, int capc,
                               int ncolours, FocusStage<T> S) {
   if (!fs->active) return;
@@ -268,6 +294,9 @@ __global__ void k_focus_stage(const FocusState* fs, SweepPtrs<T> P, BodyPtrs<T> 
     const int a = B.ba[slot], b = B.bb[slot];
     S.ba[i] = a;
     S.bb[i] = b;
+    const unsigned long long stamp = (unsigned long long)(unsigned)fs->round << 32;
+    S.la[i] = a >= 0 ? focus_local_body(S.bkey, a, stamp, &fs->nlocal, S.blist, S.capl) : -1;
+    S.lb[i] = b >= 0 ? focus_local_body(S.bkey, b, stamp, &fs->nlocal, S.blist, S.capl) : -1;
 #pragma unroll 4
     for (int e = 0; e < 36; e++) S.H[(size_t)e * S.N + i] = B.H[((size_t)s * 36 + e) * 64 + lane];
 #pragma unroll
@@ -370,25 +399,176 @@ __device__ __forceinline__ void focus_step(const FocusStage<T>& S, const BodyPtr
   }
 }
 
+// The passes with the bodies' velocities in LDS: one workgroup of
+// FOCUS_THREADS threads, thread t taking entries off[c] + t, off[c] + t + nt,
+// ... of each colour c.  A colour step is bound by the dependent arithmetic
+// of one wave's local solves, not by memory: on the 1M-contact pile 256
+// threads solve in 103 ms where 512 take 127 ms (with the velocities in HBM,
+// 113 ms), and loading the next colour's entry into registers during the
+// step (107 ms) costs more than the latency it hides.
+constexpr int FOCUS_THREADS = 256;
+constexpr int FOCUS_LDS_BYTES = 144 * 1024;
+
+// One staged entry's data, loaded before its step.
 template <typename T>
-__global__ __launch_bounds__(vel_tail_threads<T>()) void k_focus_passes_staged(
+struct FocusItem {
+  int i;  // the entry
+  int la, lb;
+  T W[9];
+  T mu;
+  typename vec4<T>::type r;
+  T pli;
+  T h[36];
+  T minv[12];
+};
+
+template <typename T>
+__device__ __forceinline__ void focus_fetch(const FocusStage<T>& S, int i, FocusItem<T>& it) {
+  it.i = i;
+  const size_t N = (size_t)S.N;
+#pragma unroll
+  for (int e = 0; e < 9; e++) it.W[e] = S.W[e * N + i];
+  it.mu = S.mu[i];
+  it.r = S.r[i];
+  it.pli = S.pli ? S.pli[i] : T(0);
+  it.la = S.la[i];
+  it.lb = S.lb[i];
+#pragma unroll
+  for (int e = 0; e < 36; e++) it.h[e] = S.H[e * N + i];
+#pragma unroll
+  for (int l = 0; l < 12; l++) it.minv[l] = S.minv[l * N + i];
+}
+
+// focus_step with the velocities in LDS (vl: six values per local body):
+// the same arithmetic in the same order, so both give the same bits.
+template <typename T>
+__device__ __forceinline__ void focus_step_lds(const FocusStage<T>& S, T* vl,
+                                               const StepConsts& sc, T tol,
+                                               const FocusItem<T>& it) {
+  SlotInputs<T> in;
+#pragma unroll
+  for (int e = 0; e < 9; e++) in.lp.W[e] = it.W[e];
+  in.lp.mu = it.mu;
+  in.ro = it.r;
+  in.pli = it.pli;
+  const int kb[2] = {it.la, it.lb};
+  T va[2][6];
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    const int k = kb[side] >= 0 ? kb[side] : 0;
+#pragma unroll
+    for (int l = 0; l < 6; l++) va[side][l] = vl[6 * k + l];
+  }
+  T u[3] = {T(0), T(0), T(0)};
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    if (kb[side] < 0) continue;
+#pragma unroll
+    for (int l = 0; l < 6; l++)
+#pragma unroll
+      for (int e = 0; e < 3; e++) u[e] += it.h[18 * side + e + 3 * l] * va[side][l];
+  }
+  const T rold[3] = {in.ro.x, in.ro.y, in.ro.z};
+  T wr[3] = {T(0), T(0), T(0)};
+  mvp3x3<T>(in.lp.W, rold, wr);
+#pragma unroll
+  for (int e = 0; e < 3; e++) in.ql[e] = u[e] - wr[e];
+  SweepPtrs<T> Pc;
+  Pc.r = S.r;
+  Pc.pli = S.pli;
+  Pc.f = nullptr;
+  typename vec4<T>::type out;
+  double light, nr2;
+  finish_slot<T>(Pc, sc, it.i >> 6, it.i & 63, tol, in, &light, &nr2, &out);
+  const T dr[3] = {out.x - rold[0], out.y - rold[1], out.z - rold[2]};
+  T vn0[6];
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    if (kb[side] < 0) continue;
+    const bool again = side == 1 && kb[1] == kb[0];
+    T vn[6];
+#pragma unroll
+    for (int l = 0; l < 6; l++) {
+      const T g = it.h[18 * side + 3 * l] * dr[0] + it.h[18 * side + 3 * l + 1] * dr[1] +
+                  it.h[18 * side + 3 * l + 2] * dr[2];
+      const T dv = it.minv[6 * side + l] * g;
+      vn[l] = (again ? vn0[l] : va[side][l]) + dv;
+    }
+#pragma unroll
+    for (int l = 0; l < 6; l++) {
+      if (side == 0) vn0[l] = vn[l];
+      vl[6 * kb[side] + l] = vn[l];
+    }
+  }
+}
+
+// The staged passes: with the velocities in LDS when the round's bodies fit
+// (use_lds and nlocal <= the LDS capacity), else in HBM (focus_step).
+template <typename T>
+__global__ __launch_bounds__(FOCUS_THREADS) void k_focus_passes_staged(
     FocusStage<T> S, BodyPtrs<T> B, const DevParams* __restrict__ prm, const SolveState* st,
     FocusState* fs, const int* __restrict__ cnt, int capc, int ncolours, int passes,
-    int want_phase) {
-  constexpr int nt = vel_tail_threads<T>();
+    int want_phase, int use_lds) {
+  constexpr int nt = FOCUS_THREADS;
+  constexpr int capb = FOCUS_LDS_BYTES / (6 * (int)sizeof(T));
   __shared__ int off[FOCUS_MAX_COLOURS + 1];
+  __shared__ int ne[FOCUS_MAX_COLOURS];  // the colours with entries, in order
+  __shared__ int nne_sh;
+  __shared__ __align__(16) T vl[6 * capb];
   if (!fs->active) return;
-  if (threadIdx.x == 0) focus_offsets(cnt, ncolours, capc, off);
+  if (threadIdx.x == 0) {
+    focus_offsets(cnt, ncolours, capc, off);
+    int n = 0;
+    if (off[0] >= 0)
+      for (int c = 0; c < ncolours; c++)
+        if (off[c + 1] > off[c]) ne[n++] = c;
+    nne_sh = n;
+  }
   __syncthreads();
   if (off[0] < 0 || off[ncolours] == 0) return;
   const StepConsts sc = focus_consts<T>(prm);
   const T tol = T(want_phase ? prm->fp32_local_tol : st->local_tol);
-  for (int pass = 0; pass < passes; pass++) {
-    for (int c = 0; c < ncolours; c++) {
-      const int b = off[c], e = off[c + 1];
-      if (b == e) continue;  // uniform
-      for (int i = b + (int)threadIdx.x; i < e; i += nt) focus_step<T>(S, B, sc, tol, i);
-      __syncthreads();
+  const int nne = nne_sh;
+  const int nloc = fs->nlocal;
+  if (use_lds && nloc <= capb) {
+    for (int id = threadIdx.x; id < nloc; id += nt) {
+      const int body = S.blist[id];
+      if (body < 0) continue;
+      const typename vec4<T>::type v0 = B.V[2 * (size_t)body], v1 = B.V[2 * (size_t)body + 1];
+      T* p = vl + 6 * id;
+      p[0] = v0.x; p[1] = v0.y; p[2] = v0.z;
+      p[3] = v1.x; p[4] = v1.y; p[5] = v1.z;
+    }
+    __syncthreads();
+    for (int pass = 0; pass < passes; pass++) {
+      for (int q = 0; q < nne; q++) {
+        const int c = ne[q];
+        for (int i = off[c] + (int)threadIdx.x; i < off[c + 1]; i += nt) {
+          FocusItem<T> x;
+          focus_fetch<T>(S, i, x);
+          focus_step_lds<T>(S, vl, sc, tol, x);
+        }
+        __syncthreads();
+      }
+    }
+    for (int id = threadIdx.x; id < nloc; id += nt) {
+      const int body = S.blist[id];
+      if (body < 0) continue;
+      const T* p = vl + 6 * id;
+      typename vec4<T>::type v0 = B.V[2 * (size_t)body], v1 = B.V[2 * (size_t)body + 1];
+      v0.x = p[0]; v0.y = p[1]; v0.z = p[2];
+      v1.x = p[3]; v1.y = p[4]; v1.z = p[5];
+      B.V[2 * (size_t)body] = v0;
+      B.V[2 * (size_t)body + 1] = v1;
+    }
+  } else {
+    for (int pass = 0; pass < passes; pass++) {
+      for (int q = 0; q < nne; q++) {
+        const int c = ne[q];
+        for (int i = off[c] + (int)threadIdx.x; i < off[c + 1]; i += nt)
+          focus_step<T>(S, B, sc, tol, i);
+        __syncthreads();
+      }
     }
   }
   if (threadIdx.x == 0) {

@@ -573,6 +573,8 @@ struct fc3d_hip_problem {
   int focus_staged = 0;
   FocusStage<float> stage32 = {};
   FocusStage<double> stage64 = {};
+  unsigned long long* d_bkey = nullptr;  // per body: round and local id (FocusStage::bkey)
+  int focus_lds = 1, graph_focus_lds[2] = {0, 0};  // 0: the passes keep velocities in HBM
 };
 
 namespace fc3d {
@@ -636,18 +638,25 @@ static int ensure_focus(fc3d_hip_problem* h, int mixed) {
     for (int c = 0; c < h->ncolours; c++)
       for (int sl = h->colour_slice_ptr[(size_t)c]; sl < h->colour_slice_ptr[(size_t)c + 1]; sl++)
         sc[(size_t)sl] = c;
-    if ((rc = dev_alloc(&h->d_fs, 1)) || (rc = dev_alloc(&h->d_bmark, (size_t)h->nbodies)) ||
-        (rc = dev_upload(&h->d_slice_colour, sc)) ||
+    if ((rc = dev_alloc(&h->d_fs, 1)) || (rc = dev_upload(&h->d_slice_colour, sc)) ||
         (rc = dev_alloc(&h->d_focus_cnt, (size_t)h->ncolours)) ||
         (rc = dev_alloc(&h->d_focus_list, (size_t)h->focus_capc * (size_t)h->ncolours)) ||
         (rc = dev_alloc(&h->d_fdiff64, (size_t)h->nslots)))
       return rc;
     FC3D_HIP_TRY(hipMemsetAsync(h->d_fs, 0, sizeof(FocusState), h->stream));
+  }
+  if (!h->d_bmark) {  // per body (set_bodies drops it with the bodies)
+    if ((rc = dev_alloc(&h->d_bmark, (size_t)h->nbodies))) return rc;
     FC3D_HIP_TRY(hipMemsetAsync(h->d_bmark, 0, sizeof(int) * (size_t)h->nbodies, h->stream));
   }
   if (mixed && !h->d_fdiff32 && (rc = dev_alloc(&h->d_fdiff32, (size_t)h->nslots))) return rc;
   h->focus_staged = h->minv_diag && h->ncolours <= FOCUS_MAX_COLOURS;
   if (h->focus_staged) {
+    if (!h->d_bkey) {
+      if ((rc = dev_alloc(&h->d_bkey, (size_t)h->nbodies))) return rc;
+      FC3D_HIP_TRY(hipMemsetAsync(h->d_bkey, 0, sizeof(unsigned long long) * (size_t)h->nbodies,
+                                  h->stream));
+    }
     const int N = h->focus_capc * h->ncolours;
     auto alloc_stage = [&](auto& S, auto* vt) -> int {
       using TT = std::remove_pointer_t<decltype(vt)>;
@@ -658,22 +667,27 @@ static int ensure_focus(fc3d_hip_problem* h, int mixed) {
           (r = dev_alloc(&S.bb, (size_t)N)) || (r = dev_alloc(&S.H, 36 * (size_t)N)) ||
           (r = dev_alloc(&S.W, 9 * (size_t)N)) || (r = dev_alloc(&S.mu, (size_t)N)) ||
           (r = dev_alloc(&S.minv, 12 * (size_t)N)) || (r = dev_alloc(&S.r, (size_t)N)) ||
-          (r = dev_alloc(&S.pli, (size_t)N)))
+          (r = dev_alloc(&S.pli, (size_t)N)) || (r = dev_alloc(&S.la, (size_t)N)) ||
+          (r = dev_alloc(&S.lb, (size_t)N)) || (r = dev_alloc(&S.blist, 2 * (size_t)N)))
         return r;
+      S.capl = 2 * N;
       (void)sizeof(TT);
       return 0;
     };
     if ((rc = alloc_stage(h->stage64, (double*)nullptr))) return rc;
     if (mixed && (rc = alloc_stage(h->stage32, (float*)nullptr))) return rc;
+    h->stage64.bkey = h->d_bkey;
+    h->stage32.bkey = h->d_bkey;
   }
   return 0;
 }
 
 static void free_stage_bufs(fc3d_hip_problem* h) {
-  void* p[] = {h->stage64.slot, h->stage64.ba, h->stage64.bb, h->stage64.H, h->stage64.W,
-               h->stage64.mu, h->stage64.minv, h->stage64.r, h->stage64.pli,
-               h->stage32.slot, h->stage32.ba, h->stage32.bb, h->stage32.H, h->stage32.W,
-               h->stage32.mu, h->stage32.minv, h->stage32.r, h->stage32.pli};
+  void* p[] = {h->stage64.slot, h->stage64.ba,  h->stage64.bb, h->stage64.H,  h->stage64.W,
+               h->stage64.mu,   h->stage64.minv, h->stage64.r, h->stage64.pli, h->stage64.la,
+               h->stage64.lb,   h->stage64.blist, h->stage32.slot, h->stage32.ba, h->stage32.bb,
+               h->stage32.H,    h->stage32.W,    h->stage32.mu, h->stage32.minv, h->stage32.r,
+               h->stage32.pli,  h->stage32.la,   h->stage32.lb, h->stage32.blist, h->d_bkey};
   for (void* q : p)
     if (q) hipFree(q);
 }
@@ -709,9 +723,9 @@ static void enqueue_focus(fc3d_hip_problem* h, int phase) {
       const SweepPtrs<float> P = ptrs32(h);
       hipLaunchKernelGGL(k_focus_stage<float>, gs, bs, 0, h->stream, h->d_fs, P, B,
                          h->d_focus_cnt, h->d_focus_list, h->focus_capc, h->ncolours, S);
-      hipLaunchKernelGGL(k_focus_passes_staged<float>, dim3(1), dim3(vel_tail_threads<float>()),
-                         0, h->stream, S, B, h->prm, h->st, h->d_fs, h->d_focus_cnt,
-                         h->focus_capc, h->ncolours, h->focus_passes, phase);
+      hipLaunchKernelGGL(k_focus_passes_staged<float>, dim3(1), dim3(FOCUS_THREADS), 0,
+                         h->stream, S, B, h->prm, h->st, h->d_fs, h->d_focus_cnt,
+                         h->focus_capc, h->ncolours, h->focus_passes, phase, h->focus_lds);
       hipLaunchKernelGGL(k_focus_unstage<float>, gs, bs, 0, h->stream, h->d_fs, P,
                          h->d_focus_cnt, h->focus_capc, h->ncolours, S);
     } else {
@@ -721,10 +735,9 @@ static void enqueue_focus(fc3d_hip_problem* h, int phase) {
       hipLaunchKernelGGL(k_focus_stage<double>, gs, bs, 0, h->stream, h->d_fs, P, B,
                          h->d_focus_cnt, h->d_focus_list, h->focus_capc, h->ncolours,
                          h->stage64);
-      hipLaunchKernelGGL(k_focus_passes_staged<double>, dim3(1),
-                         dim3(vel_tail_threads<double>()), 0, h->stream, h->stage64, B, h->prm,
-                         h->st, h->d_fs, h->d_focus_cnt, h->focus_capc, h->ncolours,
-                         h->focus_passes, phase);
+      hipLaunchKernelGGL(k_focus_passes_staged<double>, dim3(1), dim3(FOCUS_THREADS), 0,
+                         h->stream, h->stage64, B, h->prm, h->st, h->d_fs, h->d_focus_cnt,
+                         h->focus_capc, h->ncolours, h->focus_passes, phase, h->focus_lds);
       hipLaunchKernelGGL(k_focus_unstage<double>, gs, bs, 0, h->stream, h->d_fs, P,
                          h->d_focus_cnt, h->focus_capc, h->ncolours, h->stage64);
     }
@@ -1141,6 +1154,13 @@ static void drop_bodies(fc3d_hip_problem* h) {
   h->frames = 0;
   h->nbodies = 0;
   free_accel_bodies(h);
+  // the focus rounds' per-body arrays are sized by the old body count
+  if (h->d_bmark) hipFree(h->d_bmark);
+  if (h->d_bkey) hipFree(h->d_bkey);
+  h->d_bmark = nullptr;
+  h->d_bkey = nullptr;
+  h->stage32.bkey = nullptr;
+  h->stage64.bkey = nullptr;
   // captured velocity-form sweeps hold the freed pointers
   for (int ph = 0; ph < 2; ph++)
     if (h->graph[ph] && h->graph_vel[ph]) {
@@ -1213,7 +1233,8 @@ static int build_graph(fc3d_hip_problem* h, int phase, int sweeps, int schedule)
   if (h->graph[phase] && h->graph_sweeps[phase] == sweeps && h->graph_schedule[phase] == schedule &&
       h->graph_accel[phase] == h->accel_on && h->graph_vel[phase] == h->vel_on &&
       h->graph_tail[phase] == vel_tail_begin(h, phase) && h->graph_pli32_off[phase] == h->pli32_off &&
-      h->graph_freeze[phase] == h->freeze && h->graph_focus[phase] == h->focus_passes)
+      h->graph_freeze[phase] == h->freeze && h->graph_focus[phase] == h->focus_passes &&
+      h->graph_focus_lds[phase] == h->focus_lds)
     return 0;
   if (h->graph[phase]) {
     hipGraphExecDestroy(h->graph[phase]);
@@ -1234,6 +1255,7 @@ static int build_graph(fc3d_hip_problem* h, int phase, int sweeps, int schedule)
   h->graph_pli32_off[phase] = h->pli32_off;
   h->graph_freeze[phase] = h->freeze;
   h->graph_focus[phase] = h->focus_passes;
+  h->graph_focus_lds[phase] = h->focus_lds;
   return 0;
 }
 
@@ -2132,6 +2154,10 @@ int fc3d_hip_solve_observed(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_
     if ((rc = ensure_focus(h, mixed))) return rc;
     hipLaunchKernelGGL(k_focus_reset, dim3(1), dim3(64), 0, h->stream, h->d_fs);
     h->focus_passes = p->focus_passes;
+    // FC3D_HIP_FOCUS_HBM=1: the passes keep the body velocities in HBM (the
+    // same bits, slower; for comparison)
+    const char* fh = std::getenv("FC3D_HIP_FOCUS_HBM");
+    h->focus_lds = !(fh && fh[0] == '1');
   }
   if (p->velocity_form) {
     if (mixed && (rc = ensure_vel32(h))) return rc;

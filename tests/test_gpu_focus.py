@@ -4,6 +4,7 @@ whose plain sweeps spend most of their time on them.  Judged by the oracle's
 fc3d_compute_error; the selection uses atomics, so the results must not
 depend on their order (two solves give the same bits)."""
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -65,6 +66,51 @@ def test_focus_relaxation(pile, oracle, omega):
     s2 = H.solve(N.hip_params(**base, **FOCUS, focus_omega=omega))
     assert s2.iterations == s1.iterations
     np.testing.assert_array_equal(H.reaction(), r1)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_velocities_in_lds_give_the_same_bits(pile, precision):
+    """The passes keep the round's body velocities in LDS; with
+    FC3D_HIP_FOCUS_HBM=1 they read and write them in HBM instead.  The
+    arithmetic is the same, so are the results."""
+    g, H = pile
+    base = dict(tol=1e-6, max_iter=6000, warm_start=1, velocity_form=1, precision=precision,
+                fp32_tol=1e-6, fp32_local_solver=550 if precision else 0, fp32_local_tol=1e-7)
+    s1 = H.solve(N.hip_params(**base, **FOCUS))
+    r1 = H.reaction()
+    os.environ["FC3D_HIP_FOCUS_HBM"] = "1"
+    try:
+        s2 = H.solve(N.hip_params(**base, **FOCUS))
+    finally:
+        del os.environ["FC3D_HIP_FOCUS_HBM"]
+    r2 = H.reaction()
+    assert s1.info == 0 and s1.focus_rounds > 0
+    assert (s2.iterations, s2.focus_rounds, s2.focus_listed) == \
+        (s1.iterations, s1.focus_rounds, s1.focus_listed)
+    np.testing.assert_array_equal(r2, r1)
+
+
+def test_bodies_set_again_with_more_bodies(pile):
+    """set_bodies again with a larger body count (the extra bodies carry no
+    contact): the focus rounds' per-body arrays follow the new count, and
+    the solve gives the bits of the first set-up."""
+    g, H = pile
+    base = dict(tol=1e-6, max_iter=6000, warm_start=1, velocity_form=1, precision=1,
+                fp32_tol=1e-6, fp32_local_solver=550, fp32_local_tol=1e-7)
+    s1 = H.solve(N.hip_params(**base, **FOCUS))
+    r1 = H.reaction()
+    extra = 50_000
+    nb = g.nbodies + extra
+    minv = np.concatenate([g.minv, np.ones(6 * extra)])
+    vfree = np.concatenate([g.vfree, np.zeros(6 * extra)])
+    H2 = N.HipProblem(g.W, g.q, g.mu)
+    H2.set_bodies(g.nbodies, g.body_a, g.body_b, g.H, g.minv, g.vfree)
+    H2.solve(N.hip_params(**base, **FOCUS))
+    H2.set_bodies(nb, g.body_a, g.body_b, g.H, minv, vfree)
+    s2 = H2.solve(N.hip_params(**base, **FOCUS))
+    assert s2.info == 0 and s2.focus_rounds == s1.focus_rounds
+    assert s2.iterations == s1.iterations
+    np.testing.assert_array_equal(H2.reaction(), r1)
 
 
 @pytest.mark.parametrize("omega", [0.0, 2.0, float("nan")])
