@@ -176,7 +176,7 @@ __device__ inline bool aa_solve(double (*a)[AA_MAX], double* b, int n) {
   return true;
 }
 
-__global__ __launch_bounds__(1024) void k_aa_finalize(const double* __restrict__ part,
+static __global__ __launch_bounds__(1024) void k_aa_finalize(const double* __restrict__ part,
                                                       int nblocks, const SolveState* st,
                                                       AccelState* as, const DevParams* prm,
                                                       int want_phase) {

@@ -50,7 +50,7 @@ struct FocusState {
 };
 
 // At solve start: the per-solve statistics (the round mark keeps growing).
-__global__ void k_focus_reset(FocusState* fs) {
+static __global__ void k_focus_reset(FocusState* fs) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     fs->active = 0;
     fs->rounds = 0;
@@ -74,7 +74,7 @@ __device__ __forceinline__ bool focus_round_due(const DevParams* prm, const Solv
          st->iter - st->focus_last >= prm->focus_every;
 }
 
-__global__ void k_focus_clear(const DevParams* prm, const SolveState* st, FocusState* fs,
+static __global__ void k_focus_clear(const DevParams* prm, const SolveState* st, FocusState* fs,
                               int* cnt, int ncolours, int want_phase) {
   // (runs before k_focus_threshold, so it still sees the round as due)
   if (!focus_round_due(prm, st, want_phase)) return;
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(1024) void k_focus_hist(const DevParams* prm, const
 // which the suffix count reaches focus_count is the threshold.  Keys from
 // FOCUS_FINITE on hold infinities and NaNs, which no selection takes.
 constexpr int FOCUS_SCAN = 1024;
-__global__ __launch_bounds__(FOCUS_SCAN) void k_focus_threshold(const DevParams* prm,
+static __global__ __launch_bounds__(FOCUS_SCAN) void k_focus_threshold(const DevParams* prm,
                                                                 SolveState* st, FocusState* fs,
                                                                 int want_phase) {
   __shared__ long cnt[FOCUS_SCAN];
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(FOCUS_SCAN) void k_focus_threshold(const DevParams*
   fs->active = thr > 0 && total > 0.0 && top >= prm->focus_share * total;
 }
 
-__global__ void k_focus_mark(const FocusState* fs,
+static __global__ void k_focus_mark(const FocusState* fs,
                              const int* __restrict__ len, const int* __restrict__ ba,
                              const int* __restrict__ bb,
                              const float4* __restrict__ d32, const double4* __restrict__ d64,
@@ -200,7 +200,7 @@ __global__ void k_focus_mark(const FocusState* fs,
   if (bb[slot] >= 0) bmark[bb[slot]] = r;
 }
 
-__global__ void k_focus_collect(const FocusState* fs, const int* __restrict__ len, const int* __restrict__ ba,
+static __global__ void k_focus_collect(const FocusState* fs, const int* __restrict__ len, const int* __restrict__ ba,
                                 const int* __restrict__ bb, const int* __restrict__ bmark,
                                 const int* __restrict__ slice_colour, int* cnt, int* list,
                                 int capc, int nslots) {

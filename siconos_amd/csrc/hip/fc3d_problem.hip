@@ -32,6 +32,10 @@
 #include "fc3d_layout.hpp"
 #include "fc3d_mlp.hpp"
 #include "fc3d_sweep.hpp"
+#include "fc3d_fp32_kernels.hpp"
+
+// instantiated in fc3d_fp32.hip (built with FMA contraction)
+FC3D_FP32_KERNELS(extern)
 
 namespace fc3d {
 
