@@ -149,6 +149,8 @@ typedef struct fc3d_hip_stats {
   int focus_due;              /* focus rounds due (with or without passes)       */
   double focus_last_share;    /* the last due round: share of sum |dr|^2 carried *
                                * by its selection                                */
+  int focus_lds_rounds;       /* of focus_rounds, those whose body velocities fit *
+                               * in LDS (the rest kept them in HBM)              */
 } fc3d_hip_stats;
 
 FC3D_API int fc3d_hip_device_count(void);

@@ -115,7 +115,7 @@ class HipStats(C.Structure):
                 ("sweep_tail_bytes_fp32", C.c_double), ("start_trial_sweeps", C.c_int),
                 ("focus_rounds", C.c_int),
                 ("focus_listed", C.c_int64), ("focus_due", C.c_int),
-                ("focus_last_share", C.c_double)]
+                ("focus_last_share", C.c_double), ("focus_lds_rounds", C.c_int)]
 
 
 class Fc2dParams(C.Structure):

@@ -45,6 +45,8 @@ struct FocusState {
   int listed_sum; // contacts listed over those rounds
   int due;        // rounds due in this solve
   int nlocal;     // local body ids handed out by this round's k_focus_stage
+  int lds_rounds; // rounds with passes whose body velocities were in LDS
+  int pad;
   double last_share;  // the last due round: share of sum |dr|^2 its selection carried
   double last_total;  // ... and that sum
 };
@@ -56,6 +58,7 @@ static __global__ void k_focus_reset(FocusState* fs) {
     fs->rounds = 0;
     fs->listed_sum = 0;
     fs->due = 0;
+    fs->lds_rounds = 0;
     fs->last_share = 0.0;
     fs->last_total = 0.0;
   }
@@ -531,6 +534,7 @@ __global__ __launch_bounds__(FOCUS_THREADS) void k_focus_passes_staged(
   const int nne = nne_sh;
   const int nloc = fs->nlocal;
   if (use_lds && nloc <= capb) {
+    if (threadIdx.x == 0) fs->lds_rounds += 1;
     for (int id = threadIdx.x; id < nloc; id += nt) {
       const int body = S.blist[id];
       if (body < 0) continue;

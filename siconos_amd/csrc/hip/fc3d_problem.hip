@@ -2325,6 +2325,7 @@ int fc3d_hip_solve_observed(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_
     s->focus_listed = fst.listed_sum;
     s->focus_due = fst.due;
     s->focus_last_share = fst.last_share;
+    s->focus_lds_rounds = fst.lds_rounds;
   }
   s->solve_ms = ms;
   s->iterations = fin.iter;

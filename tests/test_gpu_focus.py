@@ -90,6 +90,32 @@ def test_velocities_in_lds_give_the_same_bits(pile, precision):
     np.testing.assert_array_equal(r2, r1)
 
 
+def test_rounds_beyond_the_lds_keep_their_velocities_in_hbm(pile):
+    """A round whose bodies exceed the LDS capacity (6144 FP32 velocities)
+    runs its passes with the velocities in HBM: with 1024 seed contacts the
+    rounds list about ten thousand bodies.  Those rounds give the same bits
+    as the forced HBM path, and the default selection stays in LDS."""
+    g, H = pile
+    base = dict(tol=1e-6, max_iter=6000, warm_start=1, velocity_form=1, precision=1,
+                fp32_tol=1e-6, fp32_local_solver=550, fp32_local_tol=1e-7)
+    big = dict(FOCUS, focus_count=1024)
+    s1 = H.solve(N.hip_params(**base, **big))
+    r1 = H.reaction()
+    print(f"1024 seeds: {s1.focus_rounds} rounds, {s1.focus_lds_rounds} in LDS, "
+          f"{s1.focus_listed} contacts listed", flush=True)
+    assert s1.info == 0 and s1.focus_rounds > 0
+    assert s1.focus_lds_rounds < s1.focus_rounds
+    os.environ["FC3D_HIP_FOCUS_HBM"] = "1"
+    try:
+        s2 = H.solve(N.hip_params(**base, **big))
+    finally:
+        del os.environ["FC3D_HIP_FOCUS_HBM"]
+    assert s2.focus_lds_rounds == 0 and s2.iterations == s1.iterations
+    np.testing.assert_array_equal(H.reaction(), r1)
+    s3 = H.solve(N.hip_params(**base, **FOCUS))
+    assert s3.focus_rounds > 0 and s3.focus_lds_rounds == s3.focus_rounds
+
+
 def test_bodies_set_again_with_more_bodies(pile):
     """set_bodies again with a larger body count (the extra bodies carry no
     contact): the focus rounds' per-body arrays follow the new count, and
