@@ -100,6 +100,28 @@ __device__ __forceinline__ int solve_3x3_gepp(const G* a, G* b, G* minpiv = null
   return 0;
 }
 
+// The FP32 Newton step's 3x3 solve (column-major a), by the adjugate: x =
+// adj(a) b / det(a), with no data-dependent branch, so that a wave's lanes
+// do not each take their own pivoting path.  Returns 1 without a solution
+// when det is not finite or |det| <= 2^-24 max|a_ij|^3 (a near-singular
+// matrix, which the caller solves in FP64); 0 otherwise.
+__device__ __forceinline__ int solve_3x3_adjugate(const float* a, float* b, float scale) {
+  const float c00 = a[4] * a[8] - a[7] * a[5], c01 = a[7] * a[2] - a[1] * a[8],
+              c02 = a[1] * a[5] - a[4] * a[2];
+  const float c10 = a[6] * a[5] - a[3] * a[8], c11 = a[0] * a[8] - a[6] * a[2],
+              c12 = a[3] * a[2] - a[0] * a[5];
+  const float c20 = a[3] * a[7] - a[6] * a[4], c21 = a[6] * a[1] - a[0] * a[7],
+              c22 = a[0] * a[4] - a[3] * a[1];
+  const float det = a[0] * c00 + a[3] * c01 + a[6] * c02;
+  if (!(fabsf(det) > 0x1p-24f * scale * scale * scale) || !isfinite(det)) return 1;
+  const float id = __builtin_amdgcn_rcpf(det);
+  const float b0 = b[0], b1 = b[1], b2 = b[2];
+  b[0] = (c00 * b0 + c10 * b1 + c20 * b2) * id;
+  b[1] = (c01 * b0 + c11 * b1 + c21 * b2) * id;
+  b[2] = (c02 * b0 + c12 * b1 + c22 * b2) * id;
+  return 0;
+}
+
 // projectionOnCone.c:32-50
 template <typename T>
 __device__ __forceinline__ unsigned projection_on_cone(T* r, T mu) {
@@ -447,19 +469,15 @@ __device__ int nsn_newton(const LocalProblem<T>& lp, T* r, const LocalParams& P,
     }
     d[0] = F[0]; d[1] = F[1]; d[2] = F[2];
     int info3;
-    if (sizeof(T) == 8) {
+    if constexpr (sizeof(T) == 8) {
       info3 = solve_3x3_gepp<T>(M, d);
     } else {
-      // FP32 factorisation, redone in FP64 for a near-degenerate matrix
-      // (smallest pivot below 2^-12 of the largest entry)
+      // FP32 adjugate solve, redone in FP64 for a near-degenerate matrix
       T scale = T(0);
 #pragma unroll
       for (int k = 0; k < 9; k++) scale = fmax(scale, fabs(M[k]));
-      T Mf[9], piv = T(0);
-#pragma unroll
-      for (int k = 0; k < 9; k++) Mf[k] = M[k];
-      info3 = solve_3x3_gepp<T>(Mf, d, &piv);
-      if (info3 || !(piv > T(0x1p-12) * scale)) {
+      info3 = solve_3x3_adjugate(M, d, scale);
+      if (info3) {
         double M64[9], d64[3] = {double(F[0]), double(F[1]), double(F[2])};
 #pragma unroll
         for (int k = 0; k < 9; k++) M64[k] = double(M[k]);
