@@ -250,7 +250,9 @@ FC3D_API int fc3d_hip_alart_curnier(int n, const double* R, const double* U, con
 FC3D_API int fc3d_hip_projection_on_cone(int n, double* r, const double* mu);
 /* One local solve per contact (onecontact...c:221-290, 920-1015;
  * fc3d_projection.c:350-603).  state: 4 doubles per contact [PLI rho, AC
- * rho x3] in/out (initialised from W when init_state != 0). */
+ * rho x3] in/out (initialised from W when init_state != 0).  With
+ * p->precision = 1 the FP32 sweeps' local solve runs instead
+ * (fp32_local_solver, fp32_local_tol, the inputs rounded to float). */
 FC3D_API int fc3d_hip_local_solve(int n, const double* W, const double* q, const double* mu,
                                   double* r, const fc3d_hip_params* p, double* state,
                                   int init_state, int* info);

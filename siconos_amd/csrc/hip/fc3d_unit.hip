@@ -37,26 +37,29 @@ struct UnitLocal {
   double rho_constant;
 };
 
+// T = float: the FP32 sweeps' local solve, on the inputs rounded to float;
+// results and state are widened back.
+template <typename T>
 __global__ void k_local_solve(int n, const double* W, const double* q, const double* mu,
                               double* r, UnitLocal P, double* state, int init_state, int* info) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  LocalProblem<double> lp;
-  for (int k = 0; k < 9; k++) lp.W[k] = W[9 * i + k];
-  for (int k = 0; k < 3; k++) lp.q[k] = q[3 * i + k];
-  lp.mu = mu[i];
-  double st[4];
+  LocalProblem<T> lp;
+  for (int k = 0; k < 9; k++) lp.W[k] = T(W[9 * i + k]);
+  for (int k = 0; k < 3; k++) lp.q[k] = T(q[3 * i + k]);
+  lp.mu = T(mu[i]);
+  T st[4];
   if (init_state) {
-    st[0] = 1.0;
-    compute_rho<double>(P.rho_strategy, P.rho_constant, lp.W, st + 1);
+    st[0] = T(1);
+    compute_rho<T>(P.rho_strategy, T(P.rho_constant), lp.W, st + 1);
   } else {
-    for (int k = 0; k < 4; k++) st[k] = state[4 * i + k];
+    for (int k = 0; k < 4; k++) st[k] = T(state[4 * i + k]);
   }
-  double x[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
-  double res = 0.0;
-  const int inf = local_solve<double>(lp, x, P.lp, P.tol, st, st + 1, &res);
-  for (int k = 0; k < 3; k++) r[3 * i + k] = x[k];
-  for (int k = 0; k < 4; k++) state[4 * i + k] = st[k];
+  T x[3] = {T(r[3 * i]), T(r[3 * i + 1]), T(r[3 * i + 2])};
+  T res = T(0);
+  const int inf = local_solve<T>(lp, x, P.lp, T(P.tol), st, st + 1, &res);
+  for (int k = 0; k < 3; k++) r[3 * i + k] = double(x[k]);
+  for (int k = 0; k < 4; k++) state[4 * i + k] = double(st[k]);
   info[i] = inf;
 }
 
@@ -126,6 +129,11 @@ int fc3d_hip_local_solve(int n, const double* W, const double* q, const double* 
   P.lp.hybrid_max_loop = p->hybrid_max_loop;
   P.lp.hybrid_max_iter = p->hybrid_max_iter;
   P.tol = p->local_tol;
+  const bool fp32 = p->precision == 1;
+  if (fp32) {
+    if (p->fp32_local_solver) P.lp.solver = p->fp32_local_solver;
+    P.tol = p->fp32_local_tol;
+  }
   P.rho_strategy = p->rho_strategy;
   P.rho_constant = p->rho_constant;
   double *dW, *dq, *dmu, *dr, *dst;
@@ -134,8 +142,12 @@ int fc3d_hip_local_solve(int n, const double* W, const double* q, const double* 
       (rc = upload(&dmu, mu, (size_t)n)) || (rc = upload(&dr, r, 3 * (size_t)n)) ||
       (rc = upload(&dst, state, 4 * (size_t)n)) || (rc = upload<int>(&dinfo, nullptr, (size_t)n)))
     return rc;
-  hipLaunchKernelGGL(k_local_solve, dim3((n + 63) / 64), dim3(64), 0, 0, n, dW, dq, dmu, dr, P,
-                     dst, init_state, dinfo);
+  if (fp32)
+    hipLaunchKernelGGL(k_local_solve<float>, dim3((n + 63) / 64), dim3(64), 0, 0, n, dW, dq, dmu,
+                       dr, P, dst, init_state, dinfo);
+  else
+    hipLaunchKernelGGL(k_local_solve<double>, dim3((n + 63) / 64), dim3(64), 0, 0, n, dW, dq, dmu,
+                       dr, P, dst, init_state, dinfo);
   FC3D_HIP_TRY(hipGetLastError());
   FC3D_HIP_TRY(hipMemcpy(r, dr, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost));
   FC3D_HIP_TRY(hipMemcpy(state, dst, sizeof(double) * 4 * (size_t)n, hipMemcpyDeviceToHost));
