@@ -87,8 +87,9 @@ typedef struct fc3d_hip_params {
   int freeze_contacts;
   /* Focused passes (velocity form; no reference counterpart): after every
    * focus_every sweeps from sweep focus_after on, the focus_count contacts
-   * whose reactions moved most in the last sweep are taken, if they carry at
-   * least focus_share of that sweep's sum |r_new - r_old|^2; every contact
+   * whose reactions moved most in the last sweep are taken (all that moved,
+   * if fewer did), if they carry at least focus_share of that sweep's sum
+   * |r_new - r_old|^2; every contact
    * touching one of their bodies then gets focus_passes extra Gauss-Seidel
    * passes (colour by colour, the sweep's own local step).  On granular
    * piles the last hundreds of sweeps mostly move a few small hyperstatic

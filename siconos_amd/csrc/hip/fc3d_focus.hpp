@@ -12,9 +12,10 @@
 //      (BodyPtrs::fdiff);
 //   2. k_focus_hist bins |r_new - r_old|^2 by the top bits of its FP32
 //      representation, and k_focus_threshold picks the bin at which the
-//      largest DevParams::focus_count contacts are reached; the round is
-//      skipped unless those carry focus_share of the sum (a spread-out error
-//      is the sweep's own work);
+//      largest DevParams::focus_count contacts are reached (every contact
+//      that moved at all, when fewer did); the round is skipped unless those
+//      carry focus_share of the sum (a spread-out error is the sweep's own
+//      work);
 //   3. k_focus_mark marks the bodies of the selected contacts and
 //      k_focus_collect lists, per colour, every contact touching a marked
 //      body (the clusters and their first ring);
@@ -158,7 +159,10 @@ static __global__ __launch_bounds__(FOCUS_SCAN) void k_focus_threshold(const Dev
   __syncthreads();
   if (t != 0) return;
   st->focus_last = st->iter;
-  const int thr = thr_sh;
+  // fewer than focus_count contacts moved (a local solver that stops at
+  // once on a contact already within its tolerance leaves r_new == r_old):
+  // every one that moved is taken, from the lowest nonzero bin up
+  const int thr = thr_sh < 0 && cnt[FOCUS_SCAN - 1] > 0 ? 1 : thr_sh;
   const double total = mass[FOCUS_SCAN - 1];
   double top = 0.0;
   if (thr > 0) {
