@@ -2237,9 +2237,15 @@ int fc3d_hip_solve_observed(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_
       if ((rc = reset_accel(h))) return rc;
     }
     if (h->vel_on) launch_vinit(h, phase, nullptr, nullptr);  // v from the phase's reaction
+    // The FP64 sweeps that end a mixed solve are few (2-4 at an FP32 local
+    // tolerance of 1e-7), so they replay 4 sweeps at a time: the sweeps of a
+    // replay that follow the stop still launch, and each costs about a
+    // working sweep's loads (its waves load their slots before they test the
+    // stop flag; ~140 us on 1M contacts).
+    const int ck = phase == 0 && mixed ? std::min(chunk, 4) : chunk;
     // every replay advances the sweep counter or stops; the bound only
     // guards against a launch that silently did not run
-    const long max_replays = (long)p->max_iter / (p->instrument ? 1 : chunk) + 4;
+    const long max_replays = (long)p->max_iter / (p->instrument ? 1 : ck) + 4;
     for (long replay = 0;; replay++) {
       SolveState cur;
       if (replay > max_replays) {
@@ -2271,13 +2277,13 @@ int fc3d_hip_solve_observed(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_
           }
         }
       } else if (no_graph) {
-        for (int k = 0; k < chunk; k++) enqueue_sweep(h, phase, sched[phase], nullptr, 0);
+        for (int k = 0; k < ck; k++) enqueue_sweep(h, phase, sched[phase], nullptr, 0);
         if (h->focus_passes > 0) enqueue_focus(h, phase);
         FC3D_HIP_TRY(hipGetLastError());
         FC3D_HIP_TRY(hipMemcpyAsync(&cur, h->st, sizeof cur, hipMemcpyDeviceToHost, h->stream));
         FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
       } else {
-        if ((rc = build_graph(h, phase, chunk, sched[phase]))) return rc;
+        if ((rc = build_graph(h, phase, ck, sched[phase]))) return rc;
         FC3D_HIP_TRY(hipGraphLaunch(h->graph[phase], h->stream));
         FC3D_HIP_TRY(hipMemcpyAsync(&cur, h->st, sizeof cur, hipMemcpyDeviceToHost, h->stream));
         FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
