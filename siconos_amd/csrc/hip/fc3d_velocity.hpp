@@ -145,8 +145,16 @@ __global__ void k_vinit(BodyPtrs<T> B, const typename vec4<T>::type* __restrict_
     const int s = slot >> 6, lane = slot & 63;
     const typename vec4<T>::type rc = r[slot];
     T hh[36];
-    load_h<T, FRAMES>(B, s, lane, hh);
-    const T* h = hh + 18 * side;
+    const T* h;
+    if (FRAMES) {
+      load_h<T, FRAMES>(B, s, lane, hh);
+      h = hh + 18 * side;
+    } else {
+      // only this side's 18 entries (the other side's are not needed)
+#pragma unroll
+      for (int e = 0; e < 18; e++) hh[e] = B.H[((size_t)s * 36 + 18 * side + e) * 64 + lane];
+      h = hh;
+    }
 #pragma unroll
     for (int l = 0; l < 6; l++)
       g[l] += h[3 * l] * rc.x + h[3 * l + 1] * rc.y + h[3 * l + 2] * rc.z;
