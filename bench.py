@@ -52,7 +52,7 @@ WORKLOADS = {
 # bench, so the figure is carried here with its source, measured on the same
 # kernel code as this file's.
 PMC_TRAFFIC = {
-    "pile1m": {"bytes_per_launch": 43.81e6, "fetch": 36.25e6, "write": 7.56e6,
+    "pile1m": {"bytes_per_launch": 43.81e6, "fetch": 36.25e6, "write": 7.55e6,
                "source": "profiles/k_sweep_vel_pile1m_hbm_traffic.md"},
 }
 
