@@ -197,6 +197,25 @@ def test_tail_launch_gives_the_same_iterates(lib, precision):
         assert 0 < i.sweep_tail_bytes_fp32 < i.sweep_kernel_bytes_fp32
 
 
+def test_replay_length_does_not_change_a_mixed_solve(lib):
+    """Sweeps are replayed in graphs of sweeps_per_launch sweeps (the FP64
+    sweeps that end a mixed solve at most 4 at a time); the sweeps launched
+    after the stop do nothing, so every replay length gives the same
+    iterates, hand-over sweep and result bit for bit."""
+    g = N.generate_granular_pile(20000, 50000, 0.1, 0.9, 5)
+    H = _with_bodies(g)
+    out = []
+    for spl in (1, 3, 16, 32):
+        s = H.solve(N.hip_params(tol=1e-6, max_iter=3000, warm_start=1, velocity_form=1,
+                                 precision=1, fp32_tol=1e-6, fp32_local_solver=550,
+                                 fp32_local_tol=1e-7, sweeps_per_launch=spl))
+        assert s.info == 0 and s.residual <= 1e-6
+        out.append((s.iterations, s.fp32_sweeps, s.residual, H.reaction()))
+    for it, f32, res, r in out[1:]:
+        assert (it, f32, res) == out[0][:3]
+        np.testing.assert_array_equal(r, out[0][3])
+
+
 def test_rigid_body_blocks_use_the_frames_layout_and_others_do_not(oracle):
     """With FC3D_HIP_FRAMES=1, generator piles (rigid-body blocks) take the
     frames layout (W_cc recomputed); random 3x6 blocks keep the general
