@@ -60,7 +60,9 @@ typedef struct fc3d_hip_params {
                                 start's; 3: the same against the resident
                                 reaction (previous time step)                   */
   int schedule;              /* 0: autotuned; 1..10 explicit (7: single launch)   */
-  int sweeps_per_launch;     /* sweeps captured per graph replay (>=1)            */
+  int sweeps_per_launch;     /* sweeps captured per graph replay (>=1); the FP64
+                                sweeps that end a precision = 1 solve replay at
+                                most 4 at a time                                 */
   double fp32_tol;           /* FP32 phase hands over when its light error < this */
   double fp32_local_tol;     /* local tolerance used by FP32 sweeps               */
   int instrument;            /* 1: time each sweep-kernel launch with HIP events  */
