@@ -197,9 +197,11 @@ FC3D_API int fc3d_hip_problem_set_bodies(fc3d_hip_problem* h, int nbodies, const
                                          const int* body_b, const double* H, const double* Minv,
                                          int minv_diagonal, const double* vfree);
 /* Layout of the attached factors: 0 none, 1 general 3x6 blocks of H, 2 the
- * frames layout (every contact has rigid-body blocks H_a = [T | T S(l_a)],
- * H_b = -[T | T S(l_b)]: T and the lever arms are kept, 15 values per
- * contact instead of 36, and W_cc is recomputed from them). */
+ * frames layout, taken whenever every contact has rigid-body blocks
+ * H_a = [T | T S(l_a)], H_b = -[T | T S(l_b)] (FC3D_HIP_FRAMES=0 in the
+ * environment at set_bodies keeps layout 1): T and the lever arms are kept,
+ * 15 values per contact instead of 36, and the sweeps rebuild the blocks
+ * from them; W_cc is read from W's resident diagonal blocks either way. */
 FC3D_API int fc3d_hip_problem_body_layout(const fc3d_hip_problem* h);
 FC3D_API int fc3d_hip_set_reaction(fc3d_hip_problem* h, const double* reaction);
 FC3D_API int fc3d_hip_get_reaction(const fc3d_hip_problem* h, double* reaction);

@@ -304,8 +304,17 @@ __global__ void k_focus_stage(FocusState* fs, SweepPtrs<T> P, BodyPtrs<T> B,
     const unsigned long long stamp = (unsigned long long)(unsigned)fs->round << 32;
     S.la[i] = a >= 0 ? focus_local_body(S.bkey, a, stamp, &fs->nlocal, S.blist, S.capl) : -1;
     S.lb[i] = b >= 0 ? focus_local_body(S.bkey, b, stamp, &fs->nlocal, S.blist, S.capl) : -1;
+    if (B.F) {  // frames layout: the blocks rebuilt as the sweep rebuilds them
+      T f[15], hh[36];
+#pragma unroll
+      for (int e = 0; e < 15; e++) f[e] = B.F[((size_t)s * 15 + e) * 64 + lane];
+      frames_to_h<T>(f, hh);
+#pragma unroll
+      for (int e = 0; e < 36; e++) S.H[(size_t)e * S.N + i] = hh[e];
+    } else {
 #pragma unroll 4
-    for (int e = 0; e < 36; e++) S.H[(size_t)e * S.N + i] = B.H[((size_t)s * 36 + e) * 64 + lane];
+      for (int e = 0; e < 36; e++) S.H[(size_t)e * S.N + i] = B.H[((size_t)s * 36 + e) * 64 + lane];
+    }
 #pragma unroll
     for (int e = 0; e < 9; e++) S.W[(size_t)e * S.N + i] = P.diag[((size_t)s * 9 + e) * 64 + lane];
     S.mu[i] = P.mu[slot];
@@ -588,7 +597,8 @@ __global__ __launch_bounds__(FOCUS_THREADS) void k_focus_passes_staged(
 // The passes, in one workgroup: colour after colour, a workgroup barrier
 // between colours (the next colour reads this colour's body velocities).
 // The workgroup is as large as k_sweep_vel's registers allow on one CU.
-template <typename T>
+// FRAMES: the contacts' blocks are in the frames layout.
+template <typename T, bool FRAMES>
 __global__ __launch_bounds__(vel_tail_threads<T>()) void k_focus_passes(
     SweepPtrs<T> P, BodyPtrs<T> B, const DevParams* __restrict__ prm, const SolveState* st,
     FocusState* fs, const int* __restrict__ cnt, const int* __restrict__ list, int capc,
@@ -609,8 +619,8 @@ __global__ __launch_bounds__(vel_tail_threads<T>()) void k_focus_passes(
       for (int i = threadIdx.x; i < n; i += nt) {
         const int slot = lc[i];
         double light, nr2;
-        vel_slot<T, false, false, true>(P, B, prm, st, slot >> 6, slot & 63, want_phase, light,
-                                        nr2);
+        vel_slot<T, false, FRAMES, true>(P, B, prm, st, slot >> 6, slot & 63, want_phase, light,
+                                         nr2);
       }
       __syncthreads();
     }

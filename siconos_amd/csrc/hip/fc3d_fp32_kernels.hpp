@@ -27,6 +27,9 @@
   SPEC template __global__ void fc3d::k_focus_passes_staged<float>(                              \
       fc3d::FocusStage<float>, fc3d::BodyPtrs<float>, const fc3d::DevParams*,                    \
       const fc3d::SolveState*, fc3d::FocusState*, const int*, int, int, int, int, int);          \
-  SPEC template __global__ void fc3d::k_focus_passes<float>(                                     \
+  SPEC template __global__ void fc3d::k_focus_passes<float, false>(                              \
+      fc3d::SweepPtrs<float>, fc3d::BodyPtrs<float>, const fc3d::DevParams*,                     \
+      const fc3d::SolveState*, fc3d::FocusState*, const int*, const int*, int, int, int, int);   \
+  SPEC template __global__ void fc3d::k_focus_passes<float, true>(                               \
       fc3d::SweepPtrs<float>, fc3d::BodyPtrs<float>, const fc3d::DevParams*,                     \
       const fc3d::SolveState*, fc3d::FocusState*, const int*, const int*, int, int, int, int);

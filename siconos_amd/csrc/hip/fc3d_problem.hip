@@ -750,13 +750,15 @@ static void enqueue_focus(fc3d_hip_problem* h, int phase) {
   if (phase) {
     BodyPtrs<float> B = bodies32(h);
     B.fdiff = nullptr;
-    hipLaunchKernelGGL(k_focus_passes<float>, dim3(1), dim3(vel_tail_threads<float>()), 0,
+    auto k = h->frames ? k_focus_passes<float, true> : k_focus_passes<float, false>;
+    hipLaunchKernelGGL(k, dim3(1), dim3(vel_tail_threads<float>()), 0,
                        h->stream, ptrs32(h), B, h->prm, h->st, h->d_fs, h->d_focus_cnt,
                        h->d_focus_list, h->focus_capc, h->ncolours, h->focus_passes, phase);
   } else {
     BodyPtrs<double> B = bodies64(h);
     B.fdiff = nullptr;
-    hipLaunchKernelGGL(k_focus_passes<double>, dim3(1), dim3(vel_tail_threads<double>()), 0,
+    auto k = h->frames ? k_focus_passes<double, true> : k_focus_passes<double, false>;
+    hipLaunchKernelGGL(k, dim3(1), dim3(vel_tail_threads<double>()), 0,
                        h->stream, ptrs64(h, h->part), B, h->prm, h->st, h->d_fs,
                        h->d_focus_cnt, h->d_focus_list, h->focus_capc, h->ncolours,
                        h->focus_passes, phase);
@@ -1439,9 +1441,6 @@ static bool rigid_frames(int nc, const int* body_a, const int* body_b, const dou
                          const std::vector<int>& orig2slot, int nslices, std::vector<double>* Fs) {
   Fs->assign((size_t)nslices * 15 * 64, 0.0);
   for (int c = 0; c < nc; c++) {
-    // the sweep rebuilds W_cc side by side, without the cross terms of a
-    // contact whose two sides are the same body
-    if (body_a[c] >= 0 && body_a[c] == body_b[c]) return false;
     const double* hc = H + 36 * (size_t)c;
     const bool has[2] = {body_a[c] >= 0, body_b[c] >= 0};
     double f[15] = {0};
@@ -1936,14 +1935,13 @@ int fc3d_hip_problem_set_bodies(fc3d_hip_problem* h, int nbodies, const int* bod
     bb[(size_t)slot] = body_b[c];
     for (int e = 0; e < 36; e++) Hs[((size_t)sl * 36 + e) * 64 + (size_t)lane] = H[36 * (size_t)c + e];
   }
-  // frames layout (15 values a contact instead of 45, W_cc rebuilt in the
-  // sweep) only with FC3D_HIP_FRAMES=1 in the environment and rigid-body
-  // blocks on every contact: on MI355X the sweep is bound by its chain of
-  // dependent loads, not by bytes, and the rebuild's extra M^-1 round trip
-  // makes it 9-10% slower per sweep than the general layout
+  // frames layout when every contact has rigid-body blocks: T and the lever
+  // arms (15 values a contact) instead of H (36), beside the resident W_cc
+  // (9); the sweep rebuilds H in registers.  FC3D_HIP_FRAMES=0 in the
+  // environment keeps the general layout.
   std::vector<double> Fs;
   const char* fenv = std::getenv("FC3D_HIP_FRAMES");
-  const bool frames = fenv && fenv[0] == '1' &&
+  const bool frames = !(fenv && fenv[0] == '0') &&
                       rigid_frames(nc, body_a, body_b, H, h->orig2slot, h->nslices, &Fs);
   // drop earlier factors, upload the new ones
   drop_bodies(h);
@@ -1960,19 +1958,19 @@ int fc3d_hip_problem_set_bodies(fc3d_hip_problem* h, int nbodies, const int* bod
   h->minv_diag = minv_diagonal ? 1 : 0;
   // algorithmic bytes per contact of one velocity-form colour pass, each
   // unique value counted once (the padding of the 4-wide r and v vectors is
-  // not), in values of the sweep precision: H (36) and W_cc (9), or in the
-  // frames layout T and the lever arms (15) with W_cc recomputed; mu (1), r
-  // read and written (3 + 3), PLI step read and written (2); per body side
-  // present: its velocity read and written (6 + 6) and M^-1 (6 or 36); plus
-  // len and the two body ids (12 bytes).  A two-body FP32 contact without the
-  // PLI state moves 364 bytes.
+  // not), in values of the sweep precision: H (36), or in the frames layout
+  // T and the lever arms (15); W_cc (9), mu (1), r read and written (3 + 3),
+  // PLI step read and written (2); per body side present: its velocity read
+  // and written (6 + 6) and M^-1 (6 or 36); plus len and the two body ids (12
+  // bytes).  A two-body FP32 contact without the PLI state moves 364 bytes,
+  // 280 in the frames layout.
   const double mb = minv_diagonal ? 6.0 : 36.0;
   h->vel_bytes64.assign((size_t)h->ncolours, 0.0);
   h->vel_bytes32.assign((size_t)h->ncolours, 0.0);
   for (int c = 0; c < nc; c++) {
     const int col = h->colour[(size_t)c];
     const double sides = (body_a[c] >= 0 ? 1.0 : 0.0) + (body_b[c] >= 0 ? 1.0 : 0.0);
-    const double vals = (frames ? 15 : 36 + 9) + 1 + 6 + 2 + sides * (12 + mb);
+    const double vals = (frames ? 15 : 36) + 9 + 1 + 6 + 2 + sides * (12 + mb);
     h->vel_bytes64[(size_t)col] += 8.0 * vals + 12.0;
     h->vel_bytes32[(size_t)col] += 4.0 * vals + 12.0;
   }
@@ -2071,13 +2069,13 @@ int fc3d_hip_solve_observed(fc3d_hip_problem* h, const fc3d_hip_params* p, fc3d_
     return NUMERICS_ERR_INVALID_OPTION;
   }
   if (p->focus_passes < 0 || (p->focus_passes > 0 &&
-                               (!p->velocity_form || h->frames || p->accel_depth > 0 ||
+                               (!p->velocity_form || p->accel_depth > 0 ||
                                 p->focus_every < 1 || p->focus_count < 1 || p->focus_after < 0 ||
                                 !(p->focus_share >= 0.0) || !(p->focus_omega > 0.0) ||
                                 !(p->focus_omega < 2.0)))) {
-    set_error("fc3d_hip_solve: focused passes need the velocity form (general H blocks), no "
-              "acceleration, focus_every >= 1, focus_count >= 1, focus_after >= 0, "
-              "focus_share >= 0 and 0 < focus_omega < 2");
+    set_error("fc3d_hip_solve: focused passes need the velocity form, no acceleration, "
+              "focus_every >= 1, focus_count >= 1, focus_after >= 0, focus_share >= 0 and "
+              "0 < focus_omega < 2");
     return NUMERICS_ERR_INVALID_ARGUMENT;
   }
   if (p->freeze_contacts < 0 ||

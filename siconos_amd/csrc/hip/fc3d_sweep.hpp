@@ -345,17 +345,6 @@ __device__ __forceinline__ void load_slot(const SweepPtrs<T>& P, int s, int lane
   for (int e = 0; e < 3; e++) in->ql[e] = P.q[(size_t)(s * 3 + e) * 64 + lane];
 }
 
-// The slot's state without its diagonal block: mu, the old reaction and the
-// PLI step.
-template <typename T>
-__device__ __forceinline__ void load_slot_state(const SweepPtrs<T>& P, int s, int lane,
-                                                SlotInputs<T>* in) {
-  const int slot = s * 64 + lane;
-  in->lp.mu = P.mu[slot];
-  in->ro = P.r[slot];
-  in->pli = P.pli ? P.pli[slot] : T(0);
-}
-
 // The constants of the per-contact step, read from DevParams once per wave
 // (all wave-uniform, so they sit in scalar registers).
 struct StepConsts {

@@ -91,45 +91,6 @@ __device__ __forceinline__ void load_h(const BodyPtrs<T>& B, int s, int lane, T*
   }
 }
 
-// dv = M_k^-1 g with the body's M^-1 already in registers (diagonal: mk[6]),
-// or read from memory (6x6 blocks).
-template <typename T>
-__device__ __forceinline__ void minv_apply_r(const BodyPtrs<T>& B, int k, const T* mk,
-                                             const T* g, T* dv) {
-  if (B.minv_diag) {
-#pragma unroll
-    for (int l = 0; l < 6; l++) dv[l] = mk[l] * g[l];
-  } else {
-    minv_apply<T>(B, k, g, dv);
-  }
-}
-
-// W_cc = sum over the present sides of H M^-1 H^T (column-major 3x3).
-template <typename T>
-__device__ __forceinline__ void wcc_from_h(const BodyPtrs<T>& B, const int* kb, const T (*mk)[6],
-                                           const T* h, T* W) {
-#pragma unroll
-  for (int e = 0; e < 9; e++) W[e] = T(0);
-#pragma unroll
-  for (int side = 0; side < 2; side++) {
-    if (kb[side] < 0) continue;
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      T g[6], dv[6];
-#pragma unroll
-      for (int l = 0; l < 6; l++) g[l] = h[18 * side + j + 3 * l];
-      minv_apply_r<T>(B, kb[side], mk[side], g, dv);
-#pragma unroll
-      for (int i = 0; i < 3; i++) {
-        T acc = T(0);
-#pragma unroll
-        for (int l = 0; l < 6; l++) acc += h[18 * side + i + 3 * l] * dv[l];
-        W[i + 3 * j] += acc;
-      }
-    }
-  }
-}
-
 // v_k = v_free,k + M_k^-1 sum_{c at k} H_{c,k}^T r_c for every body (in a fixed
 // order of its contacts).  With `as`, only after an Anderson step changed r.
 template <typename T, bool FRAMES>
@@ -201,35 +162,28 @@ __device__ __forceinline__ bool vel_slot(const SweepPtrs<T>& P, const BodyPtrs<T
   // FP64: the streamed planes are loaded non-temporal, which keeps V and M^-1
   // in L2 (5% a sweep); in FP32 it costs 9%
   constexpr bool NT = sizeof(T) == 8;
-  if (FRAMES)
-    load_slot_state<T>(P, s, lane, &in);  // W_cc is recomputed below
-  else
-    load_slot<T, NT>(P, s, lane, &in);
+  // W_cc is read in both layouts (the frames layout stores it beside T and
+  // the lever arms: rebuilding it needs M^-1 before the solve, one more
+  // dependent load)
+  load_slot<T, NT>(P, s, lane, &in);
   const int kb[2] = {B.ba[slot], B.bb[slot]};
   T h[36];
   load_h<T, FRAMES, NT>(B, s, lane, h);
   typename vec4<T>::type va[2][2];
-  // frames: W_cc is rebuilt from M^-1 before the local solve, so the
-  // diagonal M^-1 of both bodies is loaded here with V, not after the solve
-  T mk[2][6];
 #pragma unroll
   for (int side = 0; side < 2; side++) {
     const size_t k = kb[side] >= 0 ? (size_t)kb[side] : 0;
     va[side][0] = B.V[2 * k];
     va[side][1] = B.V[2 * k + 1];
-#pragma unroll
-    for (int l = 0; l < 6; l++) mk[side][l] = FRAMES && B.minv_diag ? B.minv[6 * k + l] : T(0);
   }
   step_consts_ready(sc);
   asm volatile("" ::"s"(tol));
   if (CHECK) {
-    if (FRAMES) asm volatile("" ::"v"(mk[0][0]), "v"(mk[1][5]));
     asm volatile("" ::"v"(len), "v"(in.lp.mu), "v"(in.ro.x), "v"(in.pli), "v"(h[0]), "v"(h[35]),
                  "v"(va[0][0].x), "v"(va[1][1].x), "s"(stop), "s"(phase));
     if (stop || phase != want_phase) return false;
   }
   if (len >= 0) {
-    if (FRAMES) wcc_from_h<T>(B, kb, mk, h, in.lp.W);
     // u = H_a v_a + H_b v_b, then q_loc = u - W_cc r_old
     T u[3] = {T(0), T(0), T(0)};
 #pragma unroll
@@ -267,7 +221,7 @@ __device__ __forceinline__ bool vel_slot(const SweepPtrs<T>& P, const BodyPtrs<T
       for (int side = 0; side < 2; side++) {
         const size_t k = kb[side] >= 0 ? (size_t)kb[side] : 0;
 #pragma unroll
-        for (int l = 0; l < 6; l++) m[side][l] = FRAMES ? mk[side][l] : B.minv[6 * k + l];
+        for (int l = 0; l < 6; l++) m[side][l] = B.minv[6 * k + l];
         if constexpr (sizeof(T) == 8) {
           vb[side][0] = B.V[2 * k];
           vb[side][1] = B.V[2 * k + 1];

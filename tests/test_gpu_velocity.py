@@ -217,19 +217,20 @@ def test_replay_length_does_not_change_a_mixed_solve(lib):
 
 
 def test_rigid_body_blocks_use_the_frames_layout_and_others_do_not(oracle):
-    """With FC3D_HIP_FRAMES=1, generator piles (rigid-body blocks) take the
-    frames layout (W_cc recomputed); random 3x6 blocks keep the general
-    layout, and so does every problem by default.  Both track the rows-of-W
+    """Generator piles (rigid-body blocks) take the frames layout (T and the
+    lever arms beside the resident W_cc) unless FC3D_HIP_FRAMES=0; random
+    3x6 blocks keep the general layout.  Both layouts track the rows-of-W
     iterates (the oracle's) to rounding."""
     import os
     g = N.generate_granular_pile(3000, 8000, 0.1, 0.9, 4)
-    assert _with_bodies(g).body_layout() == 1
-    os.environ["FC3D_HIP_FRAMES"] = "1"
+    H = _with_bodies(g)
+    assert H.body_layout() == 2
+    os.environ["FC3D_HIP_FRAMES"] = "0"
     try:
-        H = _with_bodies(g)
+        H0 = _with_bodies(g)
     finally:
         del os.environ["FC3D_HIP_FRAMES"]
-    assert H.body_layout() == 2
+    assert H0.body_layout() == 1
     rng = np.random.default_rng(11)
     nb, nc = 300, 1000
     pairs = set()
@@ -244,19 +245,15 @@ def test_rigid_body_blocks_use_the_frames_layout_and_others_do_not(oracle):
     vfree = rng.normal(size=6 * nb)
     mu = rng.uniform(0.1, 0.9, size=nc)
     W, q, _ = N.assemble(nb, ba, bb, Hr, minv, vfree)
-    os.environ["FC3D_HIP_FRAMES"] = "1"
-    try:
-        Pr = N.HipProblem(W, q, mu)
-        Pr.set_bodies(nb, ba, bb, Hr, minv, vfree)
-        assert Pr.body_layout() == 1
-    finally:
-        del os.environ["FC3D_HIP_FRAMES"]
-    for prob, kind in ((g, "pile"), (None, "random")):
+    Pr = N.HipProblem(W, q, mu)
+    Pr.set_bodies(nb, ba, bb, Hr, minv, vfree)
+    assert Pr.body_layout() == 1
+    for prob, kind in ((g, "pile"), (g, "pile, general layout"), (None, "random")):
         if prob is None:
             P = Pr
             WW, qq, mm = W, q, mu
         else:
-            P, WW, qq, mm = H, g.W, g.q, g.mu
+            P, WW, qq, mm = (H0 if "general" in kind else H), g.W, g.q, g.mu
         for K in (3, 20):
             P.solve(N.hip_params(tol=1e-300, max_iter=K, warm_start=1, velocity_form=1))
             rv = P.reaction()
@@ -319,11 +316,11 @@ def test_same_body_and_one_body_contacts(oracle):
     assert abs(P.compute_error() - s.residual) <= 1e-12 + 1e-9 * s.residual
 
 
-def test_frames_layout_refuses_a_same_body_contact(oracle):
-    """The frames layout rebuilds W_cc side by side; a contact with the same
-    body on both sides needs the cross terms, so such a problem keeps the
-    general layout even with FC3D_HIP_FRAMES=1, and stays exact."""
-    import os
+def test_frames_layout_with_a_same_body_contact(oracle):
+    """A contact with the same body on both sides: its W_cc holds the cross
+    terms of the two sides, which the frames layout reads with the other
+    diagonal blocks (it does not rebuild W_cc), so such a problem takes the
+    frames layout too and its iterates track the oracle's."""
     g = N.generate_granular_pile(300, 800, 0.1, 0.9, 4)
     ba = np.array(g.body_a, dtype=np.int32)
     bb = np.array(g.body_b, dtype=np.int32)
@@ -331,16 +328,14 @@ def test_frames_layout_refuses_a_same_body_contact(oracle):
     bb[c] = ba[c]
     Wsb, q, _ = _dense_problem(g.nbodies, ba, bb, np.asarray(g.H), np.asarray(g.minv),
                                np.asarray(g.vfree))
-    os.environ["FC3D_HIP_FRAMES"] = "1"
-    try:
-        P = N.HipProblem(Wsb, q, g.mu)
-        P.set_bodies(g.nbodies, ba, bb, g.H, g.minv, g.vfree)
-    finally:
-        del os.environ["FC3D_HIP_FRAMES"]
-    assert P.body_layout() == 1
-    P.solve(N.hip_params(tol=1e-300, max_iter=10, warm_start=1, velocity_form=1))
-    info, ro, uo, it, res = oracle.nsgs(Wsb, q, g.mu, tol=1e-300, max_iter=10, order=P.order())
-    assert np.linalg.norm(P.reaction() - ro) <= 1e-10 * np.linalg.norm(ro)
+    P = N.HipProblem(Wsb, q, g.mu)
+    P.set_bodies(g.nbodies, ba, bb, g.H, g.minv, g.vfree)
+    assert P.body_layout() == 2
+    for K in (1, 10):
+        P.solve(N.hip_params(tol=1e-300, max_iter=K, warm_start=1, velocity_form=1))
+        info, ro, uo, it, res = oracle.nsgs(Wsb, q, g.mu, tol=1e-300, max_iter=K,
+                                            order=P.order())
+        assert np.linalg.norm(P.reaction() - ro) <= 1e-10 * np.linalg.norm(ro), K
 
 
 def test_full_mass_blocks(oracle):
