@@ -318,10 +318,13 @@ __global__ void k_focus_stage(FocusState* fs, SweepPtrs<T> P, BodyPtrs<T> B,
 #pragma unroll
     for (int e = 0; e < 9; e++) S.W[(size_t)e * S.N + i] = P.diag[((size_t)s * 9 + e) * 64 + lane];
     S.mu[i] = P.mu[slot];
+    T ma[6] = {T(0), T(0), T(0), T(0), T(0), T(0)}, mb[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    if (a >= 0) minv_diagonal<T>(B, (size_t)a, ma);
+    if (b >= 0) minv_diagonal<T>(B, (size_t)b, mb);
 #pragma unroll
     for (int l = 0; l < 6; l++) {
-      S.minv[(size_t)l * S.N + i] = a >= 0 ? B.minv[6 * (size_t)a + l] : T(0);
-      S.minv[(size_t)(6 + l) * S.N + i] = b >= 0 ? B.minv[6 * (size_t)b + l] : T(0);
+      S.minv[(size_t)l * S.N + i] = ma[l];
+      S.minv[(size_t)(6 + l) * S.N + i] = mb[l];
     }
     S.r[i] = P.r[slot];
     if (S.pli) S.pli[i] = P.pli[slot];

@@ -548,6 +548,9 @@ struct fc3d_hip_problem {
   int64_t* d_block_dst = nullptr;
   int* d_o2s = nullptr;
   // velocity form (fc3d_hip_problem_set_bodies)
+  // minv_iso: every body's diagonal M^-1 is (m_t, m_t, m_t, m_r, m_r, m_r);
+  // the FP32 copy then keeps 2 values a body
+  int minv_iso = 0;
   int nbodies = 0, minv_diag = 1, vel_on = 0, graph_vel[2] = {0, 0}, graph_tail[2] = {0, 0};
   int *d_ba = nullptr, *d_bb = nullptr, *d_inc_ptr = nullptr, *d_inc = nullptr;
   double *d_H = nullptr, *d_minv = nullptr, *d_vfree = nullptr;
@@ -626,7 +629,7 @@ static BodyPtrs<double> bodies64(fc3d_hip_problem* h) {
 static BodyPtrs<float> bodies32(fc3d_hip_problem* h) {
   BodyPtrs<float> B;
   B.ba = h->d_ba; B.bb = h->d_bb; B.H = h->d_H32; B.F = h->d_F32; B.minv = h->d_minv32;
-  B.minv_diag = h->minv_diag; B.V = h->d_V32; B.vfree = h->d_vfree; B.inc_ptr = h->d_inc_ptr;
+  B.minv_diag = h->minv_diag; B.minv_iso = h->minv_iso; B.V = h->d_V32; B.vfree = h->d_vfree; B.inc_ptr = h->d_inc_ptr;
   B.inc = h->d_inc;
   if (h->focus_passes > 0) B.fdiff = h->d_fdiff32;
   return B;
@@ -1117,11 +1120,20 @@ static int ensure_accel(fc3d_hip_problem* h, int mixed) {
   return 0;
 }
 
+// Isotropic bodies: (m_t, m_r) of each body's diagonal M^-1, in FP32.
+static __global__ void k_minv_iso(const double* __restrict__ minv, float* __restrict__ m2,
+                                  int nbodies) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nbodies) return;
+  m2[2 * (size_t)k] = float(minv[6 * (size_t)k]);
+  m2[2 * (size_t)k + 1] = float(minv[6 * (size_t)k + 3]);
+}
+
 // FP32 copies of the velocity-form factors.
 static int ensure_vel32(fc3d_hip_problem* h) {
   if (h->d_H32 || h->d_F32) return 0;
   const size_t nh = (size_t)h->nslices * (h->frames ? 15 : 36) * 64;
-  const size_t nm = (size_t)h->nbodies * (h->minv_diag ? 6 : 36);
+  const size_t nm = (size_t)h->nbodies * (h->minv_iso ? 2 : h->minv_diag ? 6 : 36);
   int rc;
   float** dst = h->frames ? &h->d_F32 : &h->d_H32;
   if ((rc = dev_alloc(dst, nh)) || (rc = dev_alloc(&h->d_minv32, nm)) ||
@@ -1129,8 +1141,12 @@ static int ensure_vel32(fc3d_hip_problem* h) {
     return rc;
   hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((nh + 255) / 256)), dim3(256), 0,
                      h->stream, h->frames ? h->d_F : h->d_H, *dst, nh);
-  hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((nm + 255) / 256)), dim3(256), 0,
-                     h->stream, h->d_minv, h->d_minv32, nm);
+  if (h->minv_iso)
+    hipLaunchKernelGGL(k_minv_iso, dim3((unsigned)((h->nbodies + 255) / 256)), dim3(256), 0,
+                       h->stream, h->d_minv, h->d_minv32, h->nbodies);
+  else
+    hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((nm + 255) / 256)), dim3(256),
+                       0, h->stream, h->d_minv, h->d_minv32, nm);
   FC3D_HIP_TRY(hipGetLastError());
   FC3D_HIP_TRY(hipStreamSynchronize(h->stream));
   return 0;
@@ -1158,6 +1174,7 @@ static void drop_bodies(fc3d_hip_problem* h) {
   h->d_F = nullptr;
   h->d_F32 = nullptr;
   h->frames = 0;
+  h->minv_iso = 0;
   h->nbodies = 0;
   free_accel_bodies(h);
   // the focus rounds' per-body arrays are sized by the old body count
@@ -1956,23 +1973,34 @@ int fc3d_hip_problem_set_bodies(fc3d_hip_problem* h, int nbodies, const int* bod
   h->frames = frames ? 1 : 0;
   h->nbodies = nbodies;
   h->minv_diag = minv_diagonal ? 1 : 0;
+  // FC3D_HIP_MINV_ISO=0 in the environment keeps 6 values a body (the same
+  // numbers; for comparison)
+  const char* ienv = std::getenv("FC3D_HIP_MINV_ISO");
+  bool iso = minv_diagonal != 0 && !(ienv && ienv[0] == '0');
+  for (int k = 0; iso && k < nbodies; k++) {
+    const double* m = Minv + 6 * (size_t)k;
+    iso = m[1] == m[0] && m[2] == m[0] && m[4] == m[3] && m[5] == m[3];
+  }
+  h->minv_iso = iso ? 1 : 0;
   // algorithmic bytes per contact of one velocity-form colour pass, each
   // unique value counted once (the padding of the 4-wide r and v vectors is
   // not), in values of the sweep precision: H (36), or in the frames layout
   // T and the lever arms (15); W_cc (9), mu (1), r read and written (3 + 3),
   // PLI step read and written (2); per body side present: its velocity read
-  // and written (6 + 6) and M^-1 (6 or 36); plus len and the two body ids (12
-  // bytes).  A two-body FP32 contact without the PLI state moves 364 bytes,
-  // 280 in the frames layout.
+  // and written (6 + 6) and M^-1 (6 or 36; 2 in the FP32 copy of isotropic
+  // bodies); plus len and the two body ids (12 bytes).  A two-body FP32
+  // contact without the PLI state moves 364 bytes, 280 in the frames layout,
+  // 248 with isotropic bodies as well.
   const double mb = minv_diagonal ? 6.0 : 36.0;
+  const double mb32 = h->minv_iso ? 2.0 : mb;  // the FP32 copy of an isotropic M^-1
   h->vel_bytes64.assign((size_t)h->ncolours, 0.0);
   h->vel_bytes32.assign((size_t)h->ncolours, 0.0);
   for (int c = 0; c < nc; c++) {
     const int col = h->colour[(size_t)c];
     const double sides = (body_a[c] >= 0 ? 1.0 : 0.0) + (body_b[c] >= 0 ? 1.0 : 0.0);
-    const double vals = (frames ? 15 : 36) + 9 + 1 + 6 + 2 + sides * (12 + mb);
-    h->vel_bytes64[(size_t)col] += 8.0 * vals + 12.0;
-    h->vel_bytes32[(size_t)col] += 4.0 * vals + 12.0;
+    const double vals = (frames ? 15 : 36) + 9 + 1 + 6 + 2 + sides * 12;
+    h->vel_bytes64[(size_t)col] += 8.0 * (vals + sides * mb) + 12.0;
+    h->vel_bytes32[(size_t)col] += 4.0 * (vals + sides * mb32) + 12.0;
   }
   return 0;
 }

@@ -28,8 +28,10 @@ struct BodyPtrs {
   const int* bb;     // per slot: body of side b
   const T* H;        // [slice][36][64]: side a entries 0-17, side b 18-35 (3x6 column-major)
   const T* F;        // frames layout: [slice][15][64]: T row-major (0-8), l_a (9-11), l_b (12-14)
-  const T* minv;     // per body: 6 diagonal entries, or 36 (6x6 column-major)
+  const T* minv;     // per body: 6 diagonal entries, or 36 (6x6 column-major),
+                     // or with minv_iso 2: (m_t, m_r), M^-1 = diag(m_t I3, m_r I3)
   int minv_diag;
+  int minv_iso = 0;
   typename vec4<T>::type* V;  // per body: (v0, v1, v2), (v3, v4, v5)
   const double* vfree;        // per body: 6
   const int* inc_ptr;         // per body: range in inc
@@ -38,10 +40,26 @@ struct BodyPtrs {
   typename vec4<T>::type* fdiff = nullptr;
 };
 
+// The diagonal of body k's M^-1 (minv_diag), from 6 entries or, for bodies
+// with isotropic translational and rotational parts, from 2: the same
+// values either way.
+template <typename T>
+__device__ __forceinline__ void minv_diagonal(const BodyPtrs<T>& B, size_t k, T* m) {
+  if (B.minv_iso) {
+    const T mt = B.minv[2 * k], mr = B.minv[2 * k + 1];
+    m[0] = m[1] = m[2] = mt;
+    m[3] = m[4] = m[5] = mr;
+  } else {
+#pragma unroll
+    for (int l = 0; l < 6; l++) m[l] = B.minv[6 * k + l];
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void minv_apply(const BodyPtrs<T>& B, int k, const T* g, T* dv) {
   if (B.minv_diag) {
-    const T* m = B.minv + 6 * (size_t)k;
+    T m[6];
+    minv_diagonal<T>(B, (size_t)k, m);
 #pragma unroll
     for (int l = 0; l < 6; l++) dv[l] = m[l] * g[l];
   } else {
@@ -220,8 +238,7 @@ __device__ __forceinline__ bool vel_slot(const SweepPtrs<T>& P, const BodyPtrs<T
 #pragma unroll
       for (int side = 0; side < 2; side++) {
         const size_t k = kb[side] >= 0 ? (size_t)kb[side] : 0;
-#pragma unroll
-        for (int l = 0; l < 6; l++) m[side][l] = B.minv[6 * k + l];
+        minv_diagonal<T>(B, k, m[side]);
         if constexpr (sizeof(T) == 8) {
           vb[side][0] = B.V[2 * k];
           vb[side][1] = B.V[2 * k + 1];

@@ -216,6 +216,38 @@ def test_replay_length_does_not_change_a_mixed_solve(lib):
         np.testing.assert_array_equal(r, out[0][3])
 
 
+def test_isotropic_bodies_keep_two_values_and_the_same_bits(lib):
+    """Bodies whose diagonal M^-1 is (m_t, m_t, m_t, m_r, m_r, m_r) (the
+    generator's spheres) keep 2 values a body in the FP32 copy; the sweeps,
+    the velocity set-up and the focus stage expand them to the same six
+    numbers, so FC3D_HIP_MINV_ISO=0 (six values a body) gives the same bits."""
+    import os
+    g = N.generate_granular_pile(20000, 50000, 0.1, 0.9, 101)
+    out = []
+    for iso in ("1", "0"):
+        os.environ["FC3D_HIP_MINV_ISO"] = iso
+        try:
+            H = _with_bodies(g)
+        finally:
+            del os.environ["FC3D_HIP_MINV_ISO"]
+        rs = []
+        for K in (1, 5):
+            H.solve(N.hip_params(tol=1e-300, max_iter=K, warm_start=1, velocity_form=1,
+                                 precision=1, fp32_tol=1e-300, fp32_local_solver=550,
+                                 fp32_local_tol=1e-7))
+            rs.append(H.reaction())
+        s = H.solve(N.hip_params(tol=1e-6, max_iter=6000, warm_start=1, velocity_form=1,
+                                 precision=1, fp32_tol=1e-6, fp32_local_solver=550,
+                                 fp32_local_tol=1e-7, focus_passes=50, focus_every=16,
+                                 focus_after=64, focus_count=128, focus_share=0.5))
+        assert s.info == 0 and s.focus_rounds > 0
+        rs.append(H.reaction())
+        out.append((s.iterations, rs))
+    assert out[0][0] == out[1][0]
+    for a, b in zip(out[0][1], out[1][1]):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_rigid_body_blocks_use_the_frames_layout_and_others_do_not(oracle):
     """Generator piles (rigid-body blocks) take the frames layout (T and the
     lever arms beside the resident W_cc) unless FC3D_HIP_FRAMES=0; random
