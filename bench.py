@@ -46,14 +46,15 @@ WORKLOADS = {
 
 # HBM traffic per launch of the reported method's dominant kernel
 # (k_sweep_vel<float, true>, the frames layout the pile's rigid-body blocks
-# take), from two rocprofv3 --pmc passes (FETCH_SIZE,
+# take, with the isotropic bodies' 2-value M^-1), from two rocprofv3 --pmc
+# passes (FETCH_SIZE,
 # WRITE_SIZE) over `bench.py --steps 1 --warmup 0 --no-cpu --no-batch
 # --legs focused,plain`, scaled by the readings of a plain 1 GiB device copy
 # (FETCH x2.000, WRITE x1.000).  A profiler pass cannot run inside the timed
 # bench, so the figure is carried here with its source, measured on the same
 # kernel code as this file's.
 PMC_TRAFFIC = {
-    "pile1m": {"bytes_per_launch": 36.20e6, "fetch": 28.64e6, "write": 7.56e6,
+    "pile1m": {"bytes_per_launch": 31.57e6, "fetch": 24.05e6, "write": 7.52e6,
                "source": "profiles/k_sweep_vel_pile1m_hbm_traffic.md"},
 }
 
@@ -398,8 +399,8 @@ def main():
     pmc = PMC_TRAFFIC.get(args.workload)
     ntl = max(si.sweep_tail_launches_fp32, 1)
     frames = H.body_layout() == 2
-    layout_bytes = ("280 per two-body FP32 contact in the frames layout (T 9 and lever arms "
-                    "6 values" if frames else
+    layout_bytes = ("280 per two-body FP32 contact in the frames layout, 248 with isotropic "
+                    "bodies (T 9 and lever arms 6 values" if frames else
                     "364 per two-body FP32 contact in the general layout (H 36 values")
     roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                 "frac": achieved / 8000.0,
@@ -408,8 +409,8 @@ def main():
                 "kernel": "k_sweep_vel<float, %s> (one colour pass, velocity form)"
                           % ("true" if frames else "false"),
                 "bytes_model": "unique bytes: " + layout_bytes + ", W_cc 9, mu, r 3+3 values; "
-                               "per body v 6+6 and M^-1 6 values; len and two body ids), "
-                               "from each contact's real body count",
+                               "per body v 6+6 and M^-1 6 values, 2 for an isotropic body; "
+                               "len and two body ids), from each contact's real body count",
                 "launches": int(nl),
                 "launch_avg_us": kms * 1e3 / nl,
                 "bytes_per_launch_avg": kby / nl,
