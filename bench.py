@@ -411,6 +411,10 @@ def main():
                 "bytes_model": "unique bytes: " + layout_bytes + ", W_cc 9, mu, r 3+3 values; "
                                "per body v 6+6 and M^-1 6 values, 2 for an isotropic body; "
                                "len and two body ids), from each contact's real body count",
+                # the counters' bytes over the same launch time: the HBM rate
+                # the kernel actually draws (padding, line-granular gathers)
+                "traffic_achieved": (pmc["bytes_per_launch"] / (kms * 1e-3 / nl) / 1e9
+                                     if pmc else None),
                 "launches": int(nl),
                 "launch_avg_us": kms * 1e3 / nl,
                 "bytes_per_launch_avg": kby / nl,
