@@ -192,7 +192,10 @@ FC3D_API int fc3d_hip_problem_update(fc3d_hip_problem* h, const double* blocks, 
  * counterpart; the factors are the inputs of fc3d_hip_assemble, in the
  * caller's contact order).  Refused when the factors do not reproduce the
  * resident diagonal blocks of W, or when two contacts of one body share a
- * colour (W must hold a block for every pair of contacts sharing a body). */
+ * colour (W must hold a block for every pair of contacts sharing a body).
+ * When every body's diagonal M^-1 is (m_t, m_t, m_t, m_r, m_r, m_r) the FP32
+ * copy keeps (m_t, m_r) a body (the same numbers; FC3D_HIP_MINV_ISO=0 in the
+ * environment keeps six). */
 FC3D_API int fc3d_hip_problem_set_bodies(fc3d_hip_problem* h, int nbodies, const int* body_a,
                                          const int* body_b, const double* H, const double* Minv,
                                          int minv_diagonal, const double* vfree);
