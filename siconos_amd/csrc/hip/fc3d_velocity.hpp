@@ -15,8 +15,8 @@
 //     H_{c,a} = [ T | T S(l_a) ],   H_{c,b} = -[ T | T S(l_b) ]
 // (S(l) w = l x w; T the contact frame, l the lever arms, as the reference's
 // NewtonEuler3DR Jacobians are built), only T and the two lever arms are
-// stored (15 values instead of 36), the blocks are rebuilt in registers, and
-// W_cc = sum_k H_{c,k} M_k^-1 H_{c,k}^T is recomputed instead of read.
+// stored (15 values instead of 36) and the blocks are rebuilt in registers;
+// W_cc is read from W's diagonal blocks in both layouts.
 #pragma once
 #include "fc3d_accel.hpp"
 
