@@ -439,12 +439,13 @@ def main():
         "config": {"workload": w["desc"], "contacts": nc, "bodies": w["nbodies"],
                    "nnz_blocks": int(g.W.nnzb), "colours": H.num_colours, "tol": tol,
                    "method": "coloured NSGS from a zero start (not the learned start: see "
-                             "focused_with_learned_start) in the velocity form: FP32 sweeps "
+                             "focused_with_learned_start) in the velocity form (body layout %d: "
+                             "2 = contact frames and lever arms): FP32 sweeps "
                              "with the undamped AC Newton local solver (local tolerance 1e-7) "
                              "and focused passes (50 passes every 16 sweeps from sweep 64 over "
                              "the clusters of the 256 contacts that moved most), then FP64 "
                              "sweeps with the reference hybrid local solver to the FP64 "
-                             "residual"},
+                             "residual" % H.body_layout()},
         "focused": None, "roofline": roofline,
     }
 
