@@ -153,6 +153,28 @@ __global__ void k_vinit(BodyPtrs<T> B, const typename vec4<T>::type* __restrict_
   B.V[2 * (size_t)k + 1] = v1;
 }
 
+// g = H_k^T dr for side k of a contact: from the 3x6 blocks h, or (DIRECT:
+// the FP32 frames layout, h holding the frame T row-major and both lever
+// arms) as sg (w, w x l_k) with w = T^T dr, sg = -1 on side b.
+template <typename T, bool DIRECT>
+__device__ __forceinline__ void ht_dr(const T* h, int side, const T* dr, const T* w, T* g) {
+  if constexpr (DIRECT) {
+    const T* l = h + 9 + 3 * side;
+    const T sg = side ? T(-1) : T(1);
+    g[0] = sg * w[0];
+    g[1] = sg * w[1];
+    g[2] = sg * w[2];
+    g[3] = sg * (w[1] * l[2] - w[2] * l[1]);
+    g[4] = sg * (w[2] * l[0] - w[0] * l[2]);
+    g[5] = sg * (w[0] * l[1] - w[1] * l[0]);
+  } else {
+#pragma unroll
+    for (int l = 0; l < 6; l++)
+      g[l] = h[18 * side + 3 * l] * dr[0] + h[18 * side + 3 * l + 1] * dr[1] +
+             h[18 * side + 3 * l + 2] * dr[2];
+  }
+}
+
 // The velocity-form step of one contact, slot s * 64 + lane (s need not be
 // wave-uniform): returns false, with nothing written, when CHECK finds the
 // stop flag set; light and nr2 receive the slot's light-error term and
@@ -185,8 +207,17 @@ __device__ __forceinline__ bool vel_slot(const SweepPtrs<T>& P, const BodyPtrs<T
   // dependent load)
   load_slot<T, NT>(P, s, lane, &in);
   const int kb[2] = {B.ba[slot], B.bb[slot]};
-  T h[36];
-  load_h<T, FRAMES, NT>(B, s, lane, h);
+  // FP32 frames layout: H is never formed.  H_k v = sg T (v_lin + l_k x
+  // v_ang) and H_k^T dr = sg (w, w x l_k), w = T^T dr: fewer operations on
+  // the dependent chain and 21 registers fewer than the 36 entries of H.
+  constexpr bool DIRECT = FRAMES && sizeof(T) == 4;
+  T h[DIRECT ? 15 : 36];
+  if constexpr (DIRECT) {
+#pragma unroll
+    for (int e = 0; e < 15; e++) h[e] = B.F[((size_t)s * 15 + e) * 64 + lane];
+  } else {
+    load_h<T, FRAMES, NT>(B, s, lane, h);
+  }
   typename vec4<T>::type va[2][2];
 #pragma unroll
   for (int side = 0; side < 2; side++) {
@@ -197,8 +228,9 @@ __device__ __forceinline__ bool vel_slot(const SweepPtrs<T>& P, const BodyPtrs<T
   step_consts_ready(sc);
   asm volatile("" ::"s"(tol));
   if (CHECK) {
-    asm volatile("" ::"v"(len), "v"(in.lp.mu), "v"(in.ro.x), "v"(in.pli), "v"(h[0]), "v"(h[35]),
-                 "v"(va[0][0].x), "v"(va[1][1].x), "s"(stop), "s"(phase));
+    asm volatile("" ::"v"(len), "v"(in.lp.mu), "v"(in.ro.x), "v"(in.pli), "v"(h[0]),
+                 "v"(h[DIRECT ? 14 : 35]), "v"(va[0][0].x), "v"(va[1][1].x), "s"(stop),
+                 "s"(phase));
     if (stop || phase != want_phase) return false;
   }
   if (len >= 0) {
@@ -207,12 +239,25 @@ __device__ __forceinline__ bool vel_slot(const SweepPtrs<T>& P, const BodyPtrs<T
 #pragma unroll
     for (int side = 0; side < 2; side++) {
       if (kb[side] < 0) continue;
-      const T v[6] = {va[side][0].x, va[side][0].y, va[side][0].z,
-                      va[side][1].x, va[side][1].y, va[side][1].z};
+      if constexpr (DIRECT) {
+        const T* l = h + 9 + 3 * side;
+        const T w0 = va[side][1].x, w1 = va[side][1].y, w2 = va[side][1].z;
+        const T a[3] = {va[side][0].x + (l[1] * w2 - l[2] * w1),
+                        va[side][0].y + (l[2] * w0 - l[0] * w2),
+                        va[side][0].z + (l[0] * w1 - l[1] * w0)};
 #pragma unroll
-      for (int l = 0; l < 6; l++)
+        for (int i = 0; i < 3; i++) {
+          const T t = h[3 * i] * a[0] + h[3 * i + 1] * a[1] + h[3 * i + 2] * a[2];
+          u[i] += side ? -t : t;
+        }
+      } else {
+        const T v[6] = {va[side][0].x, va[side][0].y, va[side][0].z,
+                        va[side][1].x, va[side][1].y, va[side][1].z};
 #pragma unroll
-        for (int i = 0; i < 3; i++) u[i] += h[18 * side + i + 3 * l] * v[l];
+        for (int l = 0; l < 6; l++)
+#pragma unroll
+          for (int i = 0; i < 3; i++) u[i] += h[18 * side + i + 3 * l] * v[l];
+      }
     }
     const T rold[3] = {in.ro.x, in.ro.y, in.ro.z};
     T wr[3] = {T(0), T(0), T(0)};
@@ -233,6 +278,11 @@ __device__ __forceinline__ bool vel_slot(const SweepPtrs<T>& P, const BodyPtrs<T
     // them back with M^-1 (keeping them live through its solve spills more).
     T dv[2][6];
     typename vec4<T>::type vb[2][2];
+    T w[3] = {T(0), T(0), T(0)};  // DIRECT: T^T dr, shared by both sides
+    if constexpr (DIRECT) {
+#pragma unroll
+      for (int j = 0; j < 3; j++) w[j] = h[j] * dr[0] + h[3 + j] * dr[1] + h[6 + j] * dr[2];
+    }
     if (B.minv_diag) {
       T m[2][6];
 #pragma unroll
@@ -250,22 +300,17 @@ __device__ __forceinline__ bool vel_slot(const SweepPtrs<T>& P, const BodyPtrs<T
 #pragma unroll
       for (int side = 0; side < 2; side++) {
         if (kb[side] < 0) continue;
+        T g[6];
+        ht_dr<T, DIRECT>(h, side, dr, w, g);
 #pragma unroll
-        for (int l = 0; l < 6; l++) {
-          const T g = h[18 * side + 3 * l] * dr[0] + h[18 * side + 3 * l + 1] * dr[1] +
-                      h[18 * side + 3 * l + 2] * dr[2];
-          dv[side][l] = m[side][l] * g;
-        }
+        for (int l = 0; l < 6; l++) dv[side][l] = m[side][l] * g[l];
       }
     } else {
 #pragma unroll
       for (int side = 0; side < 2; side++) {
         if (kb[side] < 0) continue;
         T g[6];
-#pragma unroll
-        for (int l = 0; l < 6; l++)
-          g[l] = h[18 * side + 3 * l] * dr[0] + h[18 * side + 3 * l + 1] * dr[1] +
-                 h[18 * side + 3 * l + 2] * dr[2];
+        ht_dr<T, DIRECT>(h, side, dr, w, g);
         minv_apply<T>(B, kb[side], g, dv[side]);
         if constexpr (sizeof(T) == 8) {
           vb[side][0] = B.V[2 * (size_t)kb[side]];
