@@ -239,7 +239,9 @@ struct FocusStage {
   int* blist;  // [capl] body of each local id (-1: unused)
   unsigned long long* bkey;  // [nbodies] (round << 32) | local id of the body's last round
   int capl;    // 2 N: at most two ids are handed out per entry
-  T* H;        // [36][N]
+  T* H;        // [36][N]; with frames, planes 0-14 hold the frame (row-major)
+               // and both lever arms
+  int frames = 0;  // 1: the entries are in the frames layout (FP32 stage)
   T* W;        // [9][N]
   T* mu;       // [N]
   T* minv;     // [12][N]: side a 0-5, side b 6-11 (diagonal M^-1)
@@ -304,7 +306,10 @@ __global__ void k_focus_stage(FocusState* fs, SweepPtrs<T> P, BodyPtrs<T> B,
     const unsigned long long stamp = (unsigned long long)(unsigned)fs->round << 32;
     S.la[i] = a >= 0 ? focus_local_body(S.bkey, a, stamp, &fs->nlocal, S.blist, S.capl) : -1;
     S.lb[i] = b >= 0 ? focus_local_body(S.bkey, b, stamp, &fs->nlocal, S.blist, S.capl) : -1;
-    if (B.F) {  // frames layout: the blocks rebuilt as the sweep rebuilds them
+    if (S.frames) {  // the frame and the arms, as the FP32 sweep reads them
+#pragma unroll
+      for (int e = 0; e < 15; e++) S.H[(size_t)e * S.N + i] = B.F[((size_t)s * 15 + e) * 64 + lane];
+    } else if (B.F) {  // frames layout: the blocks rebuilt as the sweep rebuilds them
       T f[15], hh[36];
 #pragma unroll
       for (int e = 0; e < 15; e++) f[e] = B.F[((size_t)s * 15 + e) * 64 + lane];
@@ -362,7 +367,11 @@ __device__ __forceinline__ void focus_step(const FocusStage<T>& S, const BodyPtr
   const int kb[2] = {S.ba[i], S.bb[i]};
   T h[36];
 #pragma unroll
-  for (int e = 0; e < 36; e++) h[e] = S.H[e * N + i];
+  for (int e = 0; e < 15; e++) h[e] = S.H[e * N + i];
+  if (!S.frames) {
+#pragma unroll
+    for (int e = 15; e < 36; e++) h[e] = S.H[e * N + i];
+  }
   typename vec4<T>::type va[2][2];
 #pragma unroll
   for (int side = 0; side < 2; side++) {
@@ -376,10 +385,14 @@ __device__ __forceinline__ void focus_step(const FocusStage<T>& S, const BodyPtr
     if (kb[side] < 0) continue;
     const T v[6] = {va[side][0].x, va[side][0].y, va[side][0].z,
                     va[side][1].x, va[side][1].y, va[side][1].z};
+    if (S.frames) {
+      frames_hv<T>(h, side, v, u);
+    } else {
 #pragma unroll
-    for (int l = 0; l < 6; l++)
+      for (int l = 0; l < 6; l++)
 #pragma unroll
-      for (int e = 0; e < 3; e++) u[e] += h[18 * side + e + 3 * l] * v[l];
+        for (int e = 0; e < 3; e++) u[e] += h[18 * side + e + 3 * l] * v[l];
+    }
   }
   const T rold[3] = {in.ro.x, in.ro.y, in.ro.z};
   T wr[3] = {T(0), T(0), T(0)};
@@ -395,17 +408,19 @@ __device__ __forceinline__ void focus_step(const FocusStage<T>& S, const BodyPtr
   double light, nr2;
   finish_slot<T>(Pc, sc, i >> 6, i & 63, tol, in, &light, &nr2, &out);
   const T dr[3] = {out.x - rold[0], out.y - rold[1], out.z - rold[2]};
+  T w[3] = {T(0), T(0), T(0)};
+  if (S.frames) frames_tdr<T>(h, dr, w);
   typename vec4<T>::type vn[2][2];
 #pragma unroll
   for (int side = 0; side < 2; side++) {
     if (kb[side] < 0) continue;
-    T dv[6];
+    T g[6], dv[6];
+    if (S.frames)
+      ht_dr<T, true>(h, side, dr, w, g);
+    else
+      ht_dr<T, false>(h, side, dr, w, g);
 #pragma unroll
-    for (int l = 0; l < 6; l++) {
-      const T g = h[18 * side + 3 * l] * dr[0] + h[18 * side + 3 * l + 1] * dr[1] +
-                  h[18 * side + 3 * l + 2] * dr[2];
-      dv[l] = S.minv[(6 * side + l) * N + i] * g;
-    }
+    for (int l = 0; l < 6; l++) dv[l] = S.minv[(6 * side + l) * N + i] * g[l];
     const bool again = side == 1 && kb[1] == kb[0];
     typename vec4<T>::type v0 = again ? vn[0][0] : va[side][0];
     typename vec4<T>::type v1 = again ? vn[0][1] : va[side][1];
@@ -453,7 +468,11 @@ __device__ __forceinline__ void focus_fetch(const FocusStage<T>& S, int i, Focus
   it.la = S.la[i];
   it.lb = S.lb[i];
 #pragma unroll
-  for (int e = 0; e < 36; e++) it.h[e] = S.H[e * N + i];
+  for (int e = 0; e < 15; e++) it.h[e] = S.H[e * N + i];
+  if (!S.frames) {
+#pragma unroll
+    for (int e = 15; e < 36; e++) it.h[e] = S.H[e * N + i];
+  }
 #pragma unroll
   for (int l = 0; l < 12; l++) it.minv[l] = S.minv[l * N + i];
 }
@@ -482,10 +501,14 @@ __device__ __forceinline__ void focus_step_lds(const FocusStage<T>& S, T* vl,
 #pragma unroll
   for (int side = 0; side < 2; side++) {
     if (kb[side] < 0) continue;
+    if (S.frames) {
+      frames_hv<T>(it.h, side, va[side], u);
+    } else {
 #pragma unroll
-    for (int l = 0; l < 6; l++)
+      for (int l = 0; l < 6; l++)
 #pragma unroll
-      for (int e = 0; e < 3; e++) u[e] += it.h[18 * side + e + 3 * l] * va[side][l];
+        for (int e = 0; e < 3; e++) u[e] += it.h[18 * side + e + 3 * l] * va[side][l];
+    }
   }
   const T rold[3] = {in.ro.x, in.ro.y, in.ro.z};
   T wr[3] = {T(0), T(0), T(0)};
@@ -500,17 +523,21 @@ __device__ __forceinline__ void focus_step_lds(const FocusStage<T>& S, T* vl,
   double light, nr2;
   finish_slot<T>(Pc, sc, it.i >> 6, it.i & 63, tol, in, &light, &nr2, &out);
   const T dr[3] = {out.x - rold[0], out.y - rold[1], out.z - rold[2]};
+  T w[3] = {T(0), T(0), T(0)};
+  if (S.frames) frames_tdr<T>(it.h, dr, w);
   T vn0[6];
 #pragma unroll
   for (int side = 0; side < 2; side++) {
     if (kb[side] < 0) continue;
     const bool again = side == 1 && kb[1] == kb[0];
-    T vn[6];
+    T g[6], vn[6];
+    if (S.frames)
+      ht_dr<T, true>(it.h, side, dr, w, g);
+    else
+      ht_dr<T, false>(it.h, side, dr, w, g);
 #pragma unroll
     for (int l = 0; l < 6; l++) {
-      const T g = it.h[18 * side + 3 * l] * dr[0] + it.h[18 * side + 3 * l + 1] * dr[1] +
-                  it.h[18 * side + 3 * l + 2] * dr[2];
-      const T dv = it.minv[6 * side + l] * g;
+      const T dv = it.minv[6 * side + l] * g[l];
       vn[l] = (again ? vn0[l] : va[side][l]) + dv;
     }
 #pragma unroll

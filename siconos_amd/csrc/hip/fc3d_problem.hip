@@ -727,6 +727,7 @@ static void enqueue_focus(fc3d_hip_problem* h, int phase) {
       B.fdiff = nullptr;
       FocusStage<float> S = h->stage32;
       if (h->pli32_off) S.pli = nullptr;
+      S.frames = h->frames;  // the FP32 passes read the frames as the sweep does
       const SweepPtrs<float> P = ptrs32(h);
       hipLaunchKernelGGL(k_focus_stage<float>, gs, bs, 0, h->stream, h->d_fs, P, B,
                          h->d_focus_cnt, h->d_focus_list, h->focus_capc, h->ncolours, S);

@@ -153,6 +153,28 @@ __global__ void k_vinit(BodyPtrs<T> B, const typename vec4<T>::type* __restrict_
   B.V[2 * (size_t)k + 1] = v1;
 }
 
+// u += H_k v for side k in the frames layout (f: the frame T row-major and
+// both lever arms; v = (v_lin, v_ang)): sg T (v_lin + l_k x v_ang), sg = -1
+// on side b.
+template <typename T>
+__device__ __forceinline__ void frames_hv(const T* f, int side, const T* v, T* u) {
+  const T* l = f + 9 + 3 * side;
+  const T a[3] = {v[0] + (l[1] * v[5] - l[2] * v[4]), v[1] + (l[2] * v[3] - l[0] * v[5]),
+                  v[2] + (l[0] * v[4] - l[1] * v[3])};
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const T t = f[3 * i] * a[0] + f[3 * i + 1] * a[1] + f[3 * i + 2] * a[2];
+    u[i] += side ? -t : t;
+  }
+}
+
+// w = T^T dr in the frames layout.
+template <typename T>
+__device__ __forceinline__ void frames_tdr(const T* f, const T* dr, T* w) {
+#pragma unroll
+  for (int j = 0; j < 3; j++) w[j] = f[j] * dr[0] + f[3 + j] * dr[1] + f[6 + j] * dr[2];
+}
+
 // g = H_k^T dr for side k of a contact: from the 3x6 blocks h, or (DIRECT:
 // the FP32 frames layout, h holding the frame T row-major and both lever
 // arms) as sg (w, w x l_k) with w = T^T dr, sg = -1 on side b.
@@ -240,16 +262,9 @@ __device__ __forceinline__ bool vel_slot(const SweepPtrs<T>& P, const BodyPtrs<T
     for (int side = 0; side < 2; side++) {
       if (kb[side] < 0) continue;
       if constexpr (DIRECT) {
-        const T* l = h + 9 + 3 * side;
-        const T w0 = va[side][1].x, w1 = va[side][1].y, w2 = va[side][1].z;
-        const T a[3] = {va[side][0].x + (l[1] * w2 - l[2] * w1),
-                        va[side][0].y + (l[2] * w0 - l[0] * w2),
-                        va[side][0].z + (l[0] * w1 - l[1] * w0)};
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-          const T t = h[3 * i] * a[0] + h[3 * i + 1] * a[1] + h[3 * i + 2] * a[2];
-          u[i] += side ? -t : t;
-        }
+        const T v[6] = {va[side][0].x, va[side][0].y, va[side][0].z,
+                        va[side][1].x, va[side][1].y, va[side][1].z};
+        frames_hv<T>(h, side, v, u);
       } else {
         const T v[6] = {va[side][0].x, va[side][0].y, va[side][0].z,
                         va[side][1].x, va[side][1].y, va[side][1].z};
@@ -279,10 +294,7 @@ __device__ __forceinline__ bool vel_slot(const SweepPtrs<T>& P, const BodyPtrs<T
     T dv[2][6];
     typename vec4<T>::type vb[2][2];
     T w[3] = {T(0), T(0), T(0)};  // DIRECT: T^T dr, shared by both sides
-    if constexpr (DIRECT) {
-#pragma unroll
-      for (int j = 0; j < 3; j++) w[j] = h[j] * dr[0] + h[3 + j] * dr[1] + h[6 + j] * dr[2];
-    }
+    if constexpr (DIRECT) frames_tdr<T>(h, dr, w);
     if (B.minv_diag) {
       T m[2][6];
 #pragma unroll
