@@ -191,8 +191,10 @@ FC3D_API int fc3d_hip_problem_update(fc3d_hip_problem* h, const double* blocks, 
 /* The factors of W = H M^-1 H^T for velocity-form sweeps (no reference
  * counterpart; the factors are the inputs of fc3d_hip_assemble, in the
  * caller's contact order).  Refused when the factors do not reproduce the
- * resident diagonal blocks of W, or when two contacts of one body share a
- * colour (W must hold a block for every pair of contacts sharing a body).
+ * resident diagonal blocks of W or the resident q = H v_free (to 1e-10 of
+ * the terms' magnitude: the sweeps compute H v, so q can hold no other
+ * term), or when two contacts of one body share a colour (W must hold a
+ * block for every pair of contacts sharing a body).
  * When every body's diagonal M^-1 is (m_t, m_t, m_t, m_r, m_r, m_r) the FP32
  * copy keeps (m_t, m_r) a body (the same numbers; FC3D_HIP_MINV_ISO=0 in the
  * environment keeps six). */

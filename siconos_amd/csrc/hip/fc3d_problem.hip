@@ -1944,6 +1944,36 @@ int fc3d_hip_problem_set_bodies(fc3d_hip_problem* h, int nbodies, const int* bod
       }
     }
   }
+  // ... and the resident q: the velocity form computes a contact's local
+  // velocity as H v with v = v_free + M^-1 H^T r, which is q + W r only when
+  // q = H v_free (a q with other terms would be solved as if they were not
+  // there)
+  {
+    std::vector<double> qs((size_t)h->nslices * 3 * 64);
+    FC3D_HIP_TRY(hipMemcpy(qs.data(), h->q, sizeof(double) * qs.size(), hipMemcpyDeviceToHost));
+    for (int c = 0; c < nc; c++) {
+      const int slot = h->orig2slot[(size_t)c], sl = slot / 64, lane = slot % 64;
+      const int kb[2] = {body_a[c], body_b[c]};
+      for (int i = 0; i < 3; i++) {
+        double hv = 0.0, mag = 0.0;
+        for (int side = 0; side < 2; side++) {
+          if (kb[side] < 0) continue;
+          for (int l = 0; l < 6; l++) {
+            const double t = H[36 * (size_t)c + 18 * side + i + 3 * l] * vfree[6 * (size_t)kb[side] + l];
+            hv += t;
+            mag += std::fabs(t);
+          }
+        }
+        const double qi = qs[((size_t)sl * 3 + i) * 64 + (size_t)lane];
+        const double err = std::fabs(qi - hv);
+        if (err > 1e-10 * std::max(mag, std::fabs(qi)) + 1e-300) {
+          set_error("fc3d_hip_problem_set_bodies: H v_free differs from q at contact %d, "
+                    "component %d (%g)", c, i, err);
+          return NUMERICS_ERR_INVALID_ARGUMENT;
+        }
+      }
+    }
+  }
   // per-slot bodies and H planes
   std::vector<int> ba((size_t)h->nslots, -1), bb((size_t)h->nslots, -1);
   std::vector<double> Hs((size_t)h->nslices * 36 * 64, 0.0);

@@ -64,6 +64,13 @@ def test_factors_are_checked_and_dropped_by_updates(lib):
         H.set_bodies(g.nbodies, g.body_a, g.body_b, bad, g.minv, g.vfree)
     with pytest.raises(RuntimeError, match="-2"):
         H.set_bodies(g.nbodies, g.body_a, g.body_b + g.nbodies, g.H, g.minv, g.vfree)
+    # v_free must give the resident q = H v_free: one body's free velocity
+    # moved is refused, a rounding-level change is not
+    vf = g.vfree.copy()
+    vf[6 * int(g.body_a[0]) + 1] += 1e-3 * (1.0 + abs(vf[6 * int(g.body_a[0]) + 1]))
+    with pytest.raises(RuntimeError, match="H v_free differs from q"):
+        H.set_bodies(g.nbodies, g.body_a, g.body_b, g.H, g.minv, vf)
+    H.set_bodies(g.nbodies, g.body_a, g.body_b, g.H, g.minv, g.vfree * (1.0 + 1e-14))
     H.set_bodies(g.nbodies, g.body_a, g.body_b, g.H, g.minv, g.vfree)
     with pytest.raises(RuntimeError, match="-3"):  # the learned start needs a model
         H.solve(N.hip_params(velocity_form=1, warm_start=2))
