@@ -122,3 +122,36 @@ def test_a_star_in_the_velocity_form_with_focused_passes(oracle, star):
     velocity_form_converges(oracle, P, W, q, mu, n + 1, ba, bb, H, minv, vfree, tol=1e-8,
                             focus_passes=10, focus_every=16, focus_after=16, focus_count=16,
                             focus_share=0.0)
+
+
+@pytest.mark.parametrize("schedule", [1, 2])
+def test_the_edge_cases_in_one_batch(oracle, star, schedule):
+    """The same problems side by side in one persistent batch launch
+    (fc3d_hip_batch_*), each bit-exact with the oracle in its own sweep
+    order: a one-contact problem, a block-diagonal one, a frictionless one
+    and the star, between two ordinary problems."""
+    ba1 = np.array([0], dtype=np.int32)
+    W1, q1, *_ = random_bodies(2, ba1, np.array([1], dtype=np.int32), 11)
+    if q1[0] >= 0.0:
+        q1 = -q1
+    bad = np.arange(0, 120, 2, dtype=np.int32)
+    Wd, qd, *_ = random_bodies(120, bad, bad + 1, 12)
+    g0 = N.generate_random_pairs(80, 200, 0.0, 0.0, 3)
+    n, ba, bb, Ws, qs, H, minv, vfree, mus = star
+    rng = np.random.default_rng(5)
+    probs = N.generate_batch(2, 40, 80, 0.1, 0.9, 17)
+    probs = [probs[0], (W1, q1, np.array([0.4])), (Wd, qd, rng.uniform(0.1, 0.9, 60)),
+             (g0.W, g0.q, np.asarray(g0.mu)), (Ws, qs, mus), probs[1]]
+    B = N.Batch(probs)
+    B.set_reaction(np.zeros(3 * B.ntotal))
+    s, info, iters, res = B.solve(N.hip_params(tol=1e-8, max_iter=2000, schedule=schedule))
+    r, u, order = B.reaction(), B.velocity(), B.order()
+    for k, (W, q, mu) in enumerate(probs):
+        c0 = B.offsets[k]
+        sel = order[(order >= c0) & (order < c0 + W.nc)] - c0
+        io, ro, uo, it, reso = oracle.nsgs(W, q, mu, tol=1e-8, max_iter=2000, order=sel)
+        assert io == 0 and info[k] == io and iters[k] == it, k
+        np.testing.assert_array_equal(r[3 * c0:3 * (c0 + W.nc)], ro, err_msg=f"problem {k}")
+        np.testing.assert_array_equal(u[3 * c0:3 * (c0 + W.nc)], uo, err_msg=f"problem {k}")
+        np.testing.assert_allclose(res[k], reso, rtol=1e-12)
+    assert s.converged == len(probs)
