@@ -157,3 +157,63 @@ def test_refusals(lib, tmp_path):
     assert N.fc2d_driver(ps, r, np.zeros(4), o) == 1
     assert o.iparam[N.SICONOS_IPARAM_ITER_DONE] == 0
     assert o.dparam[N.SICONOS_DPARAM_RESIDU] == np.inf
+
+
+def _write_dense_fc2d(path, W, q, mu):
+    """A dimension-2 problem in the .dat layout of the dense fixtures
+    (dimension, contacts, storage 0, sizes, the rows of W, q, mu)."""
+    n = W.shape[0]
+    with open(path, "w") as f:
+        f.write(f"2\n{n // 2}\n0\n{n}\n{n}\n{n}\t{n}\n")
+        for row in W:
+            f.write(" ".join(f"{x:.17e}" for x in row) + "\n")
+        f.write(" ".join(f"{x:.17e}" for x in q) + "\n")
+        f.write(" ".join(f"{x:.17e}" for x in mu) + "\n")
+
+
+def _edge_problem(case):
+    rng = np.random.default_rng(7)
+    nc = 1 if case == "one contact" else 40
+    if case == "no shared body":
+        W = np.zeros((2 * nc, 2 * nc))
+        for k in range(nc):
+            A = rng.normal(size=(2, 2))
+            W[2 * k:2 * k + 2, 2 * k:2 * k + 2] = A @ A.T + 0.1 * np.eye(2)
+    else:
+        J = rng.normal(size=(2 * nc, 2 * nc + 4))
+        W = J @ J.T
+    q = rng.normal(size=2 * nc)
+    q[0::2] = -np.abs(q[0::2]) - 0.1  # every contact approaching
+    mu = np.zeros(nc) if case == "frictionless" else rng.uniform(0.1, 0.9, nc)
+    return W, q, mu
+
+
+@pytest.mark.parametrize("case", ["one contact", "no shared body", "frictionless"])
+def test_edge_cases_through_the_driver(lib, tmp_path, case):
+    """One contact; contacts whose W is block diagonal (one colour); a fully
+    coupled frictionless problem (every contact a colour of its own, the
+    tangential reactions stay 0): through fc2d_driver, against the oracle in
+    colour order, bit for bit, with both schedules the same."""
+    path = str(tmp_path / "edge.dat")
+    _write_dense_fc2d(path, *_edge_problem(case))
+    p = N.FrictionContactProblem.from_file(path)
+    nc = p.numberOfContacts
+    r, u = np.zeros(2 * nc), np.zeros(2 * nc)
+    o = _options()
+    info = N.fc2d_driver(p, r, u, o)
+    it = int(o.iparam[N.SICONOS_IPARAM_ITER_DONE])
+    H = N.Fc2dHipProblem(p)
+    order = H.order()
+    info_o, r_o, u_o, it_o, res_o = O.Fc2dProblem(path).nsgs(tol=1e-5, max_iter=10000, mode=2,
+                                                             order=order)
+    assert info == info_o == 0 and it == it_o
+    np.testing.assert_array_equal(r, r_o)
+    np.testing.assert_allclose(u, u_o, rtol=1e-13, atol=1e-13 * np.abs(u_o).max())
+    assert H.num_colours == (1 if case != "frictionless" else nc)
+    if case == "frictionless":
+        assert np.all(r[1::2] == 0.0)
+    for sched in (1, 2):
+        H.set_reaction(np.zeros(2 * nc))
+        s = H.solve(max_iter=10000, tol=1e-5, error_evaluation=2, schedule=sched)
+        assert s.info == 0 and s.iterations == it
+        np.testing.assert_array_equal(H.reaction(), r)
