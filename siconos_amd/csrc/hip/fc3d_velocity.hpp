@@ -32,7 +32,9 @@ struct BodyPtrs {
                      // or with minv_iso 2: (m_t, m_r), M^-1 = diag(m_t I3, m_r I3)
   int minv_diag;
   int minv_iso = 0;
-  typename vec4<T>::type* V;  // per body: (v0, v1, v2), (v3, v4, v5)
+  typename vec4<T>::type* V;  // per body: (v0, v1, v2, _), (v3, v4, v5, _); in FP32 with
+                              // minv_iso the fourth components hold m_t and m_r, so the
+                              // velocity gather brings M^-1 along
   const double* vfree;        // per body: 6
   const int* inc_ptr;         // per body: range in inc
   const int* inc;             // 2 slot + side, ascending slot
@@ -149,6 +151,12 @@ __global__ void k_vinit(BodyPtrs<T> B, const typename vec4<T>::type* __restrict_
   v1.y = T(B.vfree[6 * (size_t)k + 4]) + dv[4];
   v1.z = T(B.vfree[6 * (size_t)k + 5]) + dv[5];
   v1.w = T(0);
+  if (sizeof(T) == 4 && B.minv_iso) {
+    // (m_t, m_r) beside the velocity: every later writer of V keeps the
+    // fourth components as they are
+    v0.w = B.minv[2 * (size_t)k];
+    v1.w = B.minv[2 * (size_t)k + 1];
+  }
   B.V[2 * (size_t)k] = v0;
   B.V[2 * (size_t)k + 1] = v1;
 }
@@ -288,7 +296,8 @@ __device__ __forceinline__ bool vel_slot(const SweepPtrs<T>& P, const BodyPtrs<T
       B.fdiff[slot] = d;
     }
     // dv = M^-1 H^T dr for both sides.  With diagonal M^-1 both sides'
-    // entries are loaded at once, so the update waits on one round trip.
+    // entries are loaded at once, so the update waits on one round trip
+    // (none in FP32 with isotropic bodies: M^-1 rides in the velocities).
     // FP32 keeps the body velocities loaded before the solve; FP64 reads
     // them back with M^-1 (keeping them live through its solve spills more).
     T dv[2][6];
@@ -300,7 +309,14 @@ __device__ __forceinline__ bool vel_slot(const SweepPtrs<T>& P, const BodyPtrs<T
 #pragma unroll
       for (int side = 0; side < 2; side++) {
         const size_t k = kb[side] >= 0 ? (size_t)kb[side] : 0;
-        minv_diagonal<T>(B, k, m[side]);
+        if (sizeof(T) == 4 && B.minv_iso) {
+          // FP32, isotropic bodies: M^-1 came with the velocities (their
+          // fourth components), so no load follows the solve
+          m[side][0] = m[side][1] = m[side][2] = va[side][0].w;
+          m[side][3] = m[side][4] = m[side][5] = va[side][1].w;
+        } else {
+          minv_diagonal<T>(B, k, m[side]);
+        }
         if constexpr (sizeof(T) == 8) {
           vb[side][0] = B.V[2 * k];
           vb[side][1] = B.V[2 * k + 1];
