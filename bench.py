@@ -46,7 +46,8 @@ WORKLOADS = {
 
 # HBM traffic per launch of the reported method's dominant kernel
 # (k_sweep_vel<float, true>, the frames layout the pile's rigid-body blocks
-# take, with the isotropic bodies' 2-value M^-1), from two rocprofv3 --pmc
+# take, with the isotropic bodies' 2-value M^-1 carried in the velocity
+# records' fourth components), from two rocprofv3 --pmc
 # passes (FETCH_SIZE,
 # WRITE_SIZE) over `bench.py --steps 1 --warmup 0 --no-cpu --no-batch
 # --legs focused,plain`, scaled by the readings of a plain 1 GiB device copy
@@ -54,7 +55,7 @@ WORKLOADS = {
 # bench, so the figure is carried here with its source, measured on the same
 # kernel code as this file's.
 PMC_TRAFFIC = {
-    "pile1m": {"bytes_per_launch": 31.57e6, "fetch": 24.05e6, "write": 7.52e6,
+    "pile1m": {"bytes_per_launch": 28.61e6, "fetch": 21.09e6, "write": 7.52e6,
                "source": "profiles/k_sweep_vel_pile1m_hbm_traffic.md"},
 }
 
@@ -409,7 +410,8 @@ def main():
                 "kernel": "k_sweep_vel<float, %s> (one colour pass, velocity form)"
                           % ("true" if frames else "false"),
                 "bytes_model": "unique bytes: " + layout_bytes + ", W_cc 9, mu, r 3+3 values; "
-                               "per body v 6+6 and M^-1 6 values, 2 for an isotropic body; "
+                               "per body v 6+6 and M^-1 6 values, 2 for an isotropic body "
+                               "(read with v); "
                                "len and two body ids), from each contact's real body count",
                 # the counters' bytes over the same launch time: the HBM rate
                 # the kernel actually draws (padding, line-granular gathers)

@@ -196,8 +196,8 @@ FC3D_API int fc3d_hip_problem_update(fc3d_hip_problem* h, const double* blocks, 
  * term), or when two contacts of one body share a colour (W must hold a
  * block for every pair of contacts sharing a body).
  * When every body's diagonal M^-1 is (m_t, m_t, m_t, m_r, m_r, m_r) the FP32
- * copy keeps (m_t, m_r) a body (the same numbers; FC3D_HIP_MINV_ISO=0 in the
- * environment keeps six). */
+ * copy keeps (m_t, m_r) a body, in the padding of its FP32 velocity vectors
+ * (the same numbers; FC3D_HIP_MINV_ISO=0 in the environment keeps six). */
 FC3D_API int fc3d_hip_problem_set_bodies(fc3d_hip_problem* h, int nbodies, const int* body_a,
                                          const int* body_b, const double* H, const double* Minv,
                                          int minv_diagonal, const double* vfree);
