@@ -203,6 +203,84 @@ __device__ __forceinline__ void ac_std(const T* R, const T* u, T mu, const T* rh
   }
 }
 
+// ac_std in two halves, for a Newton loop that evaluates F at each new
+// iterate and needs A and B there only if it goes on: the F half keeps the
+// intermediates, and the A, B half finishes from them.  The branches and
+// the arithmetic are ac_std's.
+template <typename T>
+struct AcStdPoint {
+  T RVT, RVS, RV, Radius, RV1;
+  bool normal_active;  // RVN > 0
+};
+
+template <typename T>
+__device__ __forceinline__ void ac_std_f(const T* R, const T* u, T mu, const T* rho, T* F,
+                                         AcStdPoint<T>& p) {
+  const T RhoN = rho[0], RhoT = rho[1];
+  const T RVN = R[0] - RhoN * u[0];
+  p.RVT = R[1] - RhoT * u[1];
+  p.RVS = R[2] - RhoT * u[2];
+  p.RV = qsqrt<T>(p.RVT * p.RVT + p.RVS * p.RVS);
+  p.normal_active = RVN > T(0);
+  p.Radius = p.normal_active ? mu * RVN : T(0);
+  F[0] = p.normal_active ? RhoN * u[0] : R[0];
+  p.RV1 = T(0);
+  if (p.RV <= p.Radius) {
+    F[1] = RhoT * u[1];
+    F[2] = RhoT * u[2];
+  } else if (p.RV > p.Radius) {
+    if (p.Radius > T(0)) {
+      p.RV1 = qdiv<T>(T(1), p.RV);
+      F[1] = R[1] - p.Radius * p.RVT * p.RV1;
+      F[2] = R[2] - p.Radius * p.RVS * p.RV1;
+    } else {
+      F[1] = R[1];
+      F[2] = R[2];
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void ac_std_ab(const AcStdPoint<T>& p, T mu, const T* rho, T* A,
+                                          T* B) {
+  const T RhoN = rho[0], RhoT = rho[1];
+  const T RVT = p.RVT, RVS = p.RVS, RV = p.RV, Radius = p.Radius, RV1 = p.RV1;
+  A[3] = T(0); A[6] = T(0); A[1] = T(0); A[2] = T(0);
+  B[3] = T(0); B[6] = T(0); B[1] = T(0); B[2] = T(0);
+  if (p.normal_active) {
+    A[0] = RhoN; B[0] = T(0);
+  } else {
+    A[0] = T(0); B[0] = T(1);
+  }
+  if (RV <= Radius) {
+    A[4] = RhoT; A[7] = T(0); A[5] = T(0); A[8] = RhoT;
+    B[1] = T(0); B[4] = T(0); B[7] = T(0); B[2] = T(0); B[5] = T(0); B[8] = T(0);
+  } else if (RV > Radius) {
+    if (Radius > T(0)) {
+      const T RV3 = RV1 * RV1 * RV1;
+      const T GTT = RV1 - RVT * RVT * RV3;
+      const T GTS = -RVT * RVS * RV3;
+      const T GST = GTS;
+      const T GSS = RV1 - RVS * RVS * RV3;
+      A[1] = mu * RhoN * RVT * RV1;
+      A[2] = mu * RhoN * RVS * RV1;
+      A[4] = GTT * RhoT * Radius;
+      A[7] = GTS * RhoT * Radius;
+      A[5] = GST * RhoT * Radius;
+      A[8] = GSS * RhoT * Radius;
+      B[1] = -mu * RVT * RV1;
+      B[4] = T(1) - GTT * Radius;
+      B[7] = -GTS * Radius;
+      B[2] = -mu * RVS * RV1;
+      B[5] = -GST * Radius;
+      B[8] = T(1) - GSS * Radius;
+    } else {
+      A[4] = T(0); A[7] = T(0); A[5] = T(0); A[8] = T(0);
+      B[1] = T(0); B[4] = T(1); B[7] = T(0); B[2] = T(0); B[5] = T(0); B[8] = T(1);
+    }
+  }
+}
+
 // fc3d_AlartCurnier_functions.c:649-753 (radius mu*R[0])
 template <typename T, bool AB>
 __device__ __forceinline__ void ac_jeanmoreau(const T* R, const T* u, T mu, const T* rho, T* F,
@@ -439,10 +517,88 @@ __device__ void line_search_gp(const LocalProblem<T>& lp, int formulation, T* t_
   *t_opt = alpha;
 }
 
+// The Newton direction of onecontact...c:500-521 (and 786-805): solve
+// -(A W + B) dR = F; dR = 0 when the matrix is singular (the returned info).
+template <typename T>
+__device__ __forceinline__ int newton_direction(const T* A, const T* B, const T* W, const T* F,
+                                                T* dR) {
+  T AW[9];
+  mm3x3<T>(A, W, AW);
+  T M[9], d[3];
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    T x = AW[k] + B[k];
+    x = x * T(-1);
+    M[k] = x;
+  }
+  d[0] = F[0]; d[1] = F[1]; d[2] = F[2];
+  int info3;
+  if constexpr (sizeof(T) == 8) {
+    info3 = solve_3x3_gepp<T>(M, d);
+  } else {
+    // FP32 adjugate solve, redone in FP64 for a near-degenerate matrix
+    T scale = T(0);
+#pragma unroll
+    for (int k = 0; k < 9; k++) scale = fmax(scale, fabs(M[k]));
+    info3 = solve_3x3_adjugate(M, d, scale);
+    if (info3) {
+      double M64[9], d64[3] = {double(F[0]), double(F[1]), double(F[2])};
+#pragma unroll
+      for (int k = 0; k < 9; k++) M64[k] = double(M[k]);
+      info3 = solve_3x3_gepp<double>(M64, d64);
+      d[0] = T(d64[0]); d[1] = T(d64[1]); d[2] = T(d64[2]);
+    }
+  }
+  if (info3) {
+    dR[0] = T(0); dR[1] = T(0); dR[2] = T(0);
+  } else {
+    dR[0] = T(d[0]); dR[1] = T(d[1]); dR[2] = T(d[2]);
+  }
+  return info3;
+}
+
+// nsn_newton's undamped loop (onecontact...c:429-559) for the FP32 sweeps
+// with the STD formulation: the same steps, except that the A and B of an
+// iteration (the evaluation at :498) are finished from the intermediates of
+// the F evaluation that tested the same iterate (:480, :528), instead of a
+// second evaluation of the function there.
+template <typename T>
+__device__ int nsn_newton_std_fp32(const LocalProblem<T>& lp, T* r, const LocalParams& P, T tol,
+                                   const T* rho, T* residual) {
+  const T nq = qsqrt<T>(lp.q[0] * lp.q[0] + lp.q[1] * lp.q[1] + lp.q[2] * lp.q[2]);
+  T nrel = T(1);
+  if (nq > num<T>::eps) nrel = qdiv<T>(nrel, nq);
+  T dR[3] = {T(0), T(0), T(0)}, F[3], v[3];
+  T A[9] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+  T B[9] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+  AcStdPoint<T> pt;
+  local_velocity<T>(lp, r, v);
+  ac_std_f<T>(r, v, lp.mu, rho, F, pt);
+  *residual = qsqrt<T>(F[0] * F[0] + F[1] * F[1] + F[2] * F[2]) * nrel;
+  if (*residual < tol) return 0;
+  for (int inew = 1; inew < P.newton_max_iter; ++inew) {
+    ac_std_ab<T>(pt, lp.mu, rho, A, B);
+    const int info3 = newton_direction<T>(A, B, lp.W, F, dR);
+    r[0] += dR[0]; r[1] += dR[1]; r[2] += dR[2];
+    local_velocity<T>(lp, r, v);
+    ac_std_f<T>(r, v, lp.mu, rho, F, pt);
+    *residual = qsqrt<T>(F[0] * F[0] + F[1] * F[1] + F[2] * F[2]) * nrel;
+    const T ndR = qsqrt<T>(dR[0] * dR[0] + dR[1] * dR[1] + dR[2] * dR[2]);
+    if (*residual < tol) return 0;
+    if (ndR < T(10) * num<T>::eps) return 1;
+    if (info3) break;
+  }
+  return 1;
+}
+
 // onecontact...c:429-559 (damped = false) and 710-854 (damped = true)
 template <typename T>
 __device__ int nsn_newton(const LocalProblem<T>& lp, T* r, const LocalParams& P, T tol,
                           const T* rho, bool damped, T* residual) {
+  if constexpr (sizeof(T) == 4) {
+    if (!damped && P.formulation != 1)
+      return nsn_newton_std_fp32<T>(lp, r, P, tol, rho, residual);
+  }
   const T nq = qsqrt<T>(lp.q[0] * lp.q[0] + lp.q[1] * lp.q[1] + lp.q[2] * lp.q[2]);
   T nrel = T(1);
   if (nq > num<T>::eps) nrel = qdiv<T>(nrel, nq);
